@@ -1,0 +1,154 @@
+/*
+ * mdzg.h -- C ABI of the MI355X (gfx950) escape-time engine for MDZ.
+ *
+ * The library replaces MDZ's per-pixel escape-time hot path:
+ *   the formula loops  frac_X / frac_X_mpfr       (src/frac_mandel.h:12-23,
+ *                      src/frac_burning_ship.h:12-23, src/frac_generalized_celtic.h:12-23,
+ *                      src/frac_variant.h:12-23)
+ *   the line workers   fractal_calculate_line / fractal_mpfr_calculate_line
+ *                      (src/fractal.h:42-43, src/fractal.c:29-257)
+ * and sits under the line pool of src/render_threads.c (the next_line_cb of
+ * src/render_threads.h:70-71) and the command-line render path
+ * (render_to_file, src/render.c:10-105).
+ *
+ * Results are bit-identical to the reference loops: the MPFR path rounds every
+ * operation to nearest-even at the view's precision exactly as MPFR_RNDN does,
+ * the long double path equals the x87 loop (round-to-nearest at 64 bits), and
+ * the double path equals the same C loop evaluated in IEEE double.
+ *
+ * Plain C types only: views are described by the image_info fields the line
+ * workers read, coordinates as MPFR-parsable strings (decimal or the hex-float
+ * form mpfr_asprintf("%Ra") prints, which is exact).
+ *
+ * Return values: >= 0 success, negative MDZG_ERR_* codes.  Every entry point
+ * is safe to call from several host threads on one context.
+ */
+#ifndef MDZG_H
+#define MDZG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDZG_ABI_VERSION 1
+
+enum {
+    MDZG_OK = 0,
+    MDZG_ERR_ARG = -1,      /* invalid argument                         */
+    MDZG_ERR_NOMEM = -2,    /* host or device allocation failed         */
+    MDZG_ERR_DEVICE = -3,   /* HIP runtime error                        */
+    MDZG_ERR_STOPPED = -4,  /* render stopped before completion         */
+    MDZG_ERR_BUSY = -5,     /* admission refused (budget exhausted)     */
+    MDZG_ERR_PARSE = -6,    /* a coordinate string did not parse        */
+    MDZG_ERR_RANGE = -7,    /* precision or size beyond what is built   */
+    MDZG_ERR_STATE = -8     /* call out of sequence                     */
+};
+
+/* fractal types, same values as the enum of src/fractal.h:17-26 */
+enum { MDZG_MANDELBROT = 0, MDZG_BURNING_SHIP = 1, MDZG_GENERALIZED_CELTIC = 2,
+       MDZG_VARIANT = 3 };
+/* families, same values as src/fractal.h:8-15 */
+enum { MDZG_FAMILY_MANDEL = 0, MDZG_FAMILY_JULIA = 1 };
+/* arithmetic, the choice image_info_set_multi_prec makes (src/image_info.c:226-249):
+   MPFR = fractal_mpfr_calculate_line, LD = fractal_calculate_line (long double),
+   DOUBLE = the same loop in IEEE double (the FP64 path) */
+enum { MDZG_PATH_MPFR = 0, MDZG_PATH_LD = 1, MDZG_PATH_DOUBLE = 2 };
+
+#define MDZG_MAX_PRECISION 1024   /* largest p of the MPFR path built for the device */
+
+/* The image_info fields the line workers read (src/image_info.h:63-122). */
+typedef struct mdzg_view {
+    int fractal;            /* image_info.fractal                                 */
+    int family;             /* image_info.family                                  */
+    int path;               /* MDZG_PATH_*                                        */
+    int real_width;         /* image_info.real_width  (user width x aa)           */
+    int real_height;        /* image_info.real_height                             */
+    long precision;         /* image_info.precision: p of the MPFR path           */
+    long long depth;        /* image_info.depth (1 .. INT32_MAX)                  */
+    const char* xmin;       /* image_info.xmin, .xmax, .ymax, .width (held at     */
+    const char* xmax;       /*   max(p, 80) bits, as image_info_set_precision     */
+    const char* ymax;       /*   keeps them, src/image_info.c:252-280)            */
+    const char* width;
+    const char* julia_re;   /* image_info.u.julia.c_re / c_im (Julia family only) */
+    const char* julia_im;
+} mdzg_view;
+
+/* A rectangle of a view: rows [line0, line0+nlines) x columns [col0, col0+ncols),
+   counts written row-major (ncols per row). */
+typedef struct mdzg_tile {
+    mdzg_view view;
+    int line0, nlines;
+    int col0, ncols;
+    uint32_t* counts;       /* host memory, nlines*ncols                          */
+} mdzg_tile;
+
+/* Timings and work of the last batch (device clock, HIP events). */
+typedef struct mdzg_stats {
+    double kernel_ms;       /* first launch start to last launch end on the device */
+    double wall_ms;         /* host wall time of the call, staging and copies incl. */
+    double pixel_iters;     /* sum over pixels of (count ? count : depth)          */
+    int launches;           /* kernel launches made                               */
+} mdzg_stats;
+
+typedef struct mdzg_ctx mdzg_ctx;
+
+int         mdzg_abi_version(void);
+const char* mdzg_strerror(int code);
+
+/* Open device `device` (HIP ordinal).  One context per process per device. */
+int  mdzg_create(int device, mdzg_ctx** out);
+void mdzg_destroy(mdzg_ctx* ctx);
+
+/* Rect of a centre/size view as the command-line path computes it
+   (coords_set_precision src/coords.c:204-231, coords_center_to_rect :265-302,
+   coords_get_rect :305-313).  Writes img->xmin/xmax/ymax/width as hex-float
+   strings at max(precision, 80) bits into buffers of `cap` bytes. */
+int mdzg_rect_from_center(const char* cx, const char* cy, const char* size,
+                          long precision, int user_w, int user_h,
+                          char* xmin, char* xmax, char* ymax, char* width, int cap);
+
+/* Synchronous render of one tile: what fractal_*_calculate_line computes for
+   those rows and columns (src/fractal.c:29-257), in one persistent launch. */
+int mdzg_render_tile(mdzg_ctx* ctx, const mdzg_tile* tile, mdzg_stats* stats);
+
+/* Batch modes */
+enum { MDZG_BATCH_PERSISTENT = 0,   /* one persistent launch per arithmetic class,
+                                       draining a device queue of all tiles' pixels */
+       MDZG_BATCH_PER_TILE = 1 };   /* one plain launch per tile (baseline form) */
+
+/* Render many tiles (uneven size, precision, depth, formula).  Counts are
+   identical in both modes. */
+int mdzg_render_batch(mdzg_ctx* ctx, const mdzg_tile* tiles, int ntiles, int mode,
+                      mdzg_stats* stats);
+
+/* ---- asynchronous form (submit / poll / wait / cancel) ---------------- */
+typedef struct mdzg_job mdzg_job;
+int  mdzg_submit(mdzg_ctx* ctx, const mdzg_tile* tiles, int ntiles, mdzg_job** out);
+int  mdzg_poll(mdzg_job* job);      /* 1 done, 0 running, < 0 error             */
+int  mdzg_wait(mdzg_job* job, mdzg_stats* stats); /* MDZG_OK or MDZG_ERR_STOPPED */
+int  mdzg_cancel(mdzg_job* job);    /* device-side stop; untouched counts keep
+                                       their previous contents                  */
+void mdzg_job_free(mdzg_job* job);
+
+/* ---- line-callback adapter ---------------------------------------------
+   A frame holds one view and the caller's raw buffer (image_info.raw_data,
+   real_width*real_height ints, src/image_info.c:136-137).  The frame is
+   rendered on the device in row bands; mdzg_frame_calculate_line has the
+   shape and return convention of fractal_mpfr_calculate_line (src/fractal.h:31-44):
+   it fills row `line` of raw and returns 1, or returns 0 once the frame was
+   stopped (rows never rendered keep whatever raw held, e.g. the UINT_MAX
+   sentinel of src/image_info.c:214-215).  Safe to call from several worker
+   threads at once with distinct lines, as src/render_threads.c:376 does. */
+typedef struct mdzg_frame mdzg_frame;
+int  mdzg_frame_open(mdzg_ctx* ctx, const mdzg_view* view, int32_t* raw, mdzg_frame** out);
+int  mdzg_frame_calculate_line(mdzg_frame* frame, int line);
+void mdzg_frame_stop(mdzg_frame* frame);    /* the RT_STOP of src/render_threads.c:409-418 */
+int  mdzg_frame_close(mdzg_frame* frame, mdzg_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

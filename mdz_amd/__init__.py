@@ -1,0 +1,236 @@
+"""mdz_amd -- MI355X (gfx950) escape-time engine for MDZ, Python binding.
+
+A thin ctypes layer over the C ABI of include/mdzg.h (libmdzg.so, built in
+tree by mdz_amd.build).  Loading fails loudly when the library is missing:
+there is no CPU path behind these calls.
+
+Names follow the reference: a *view* is the set of image_info fields the line
+workers read (src/image_info.h:63-122), a *tile* a rectangle of rows and
+columns of a view, counts are the per-pixel iteration counts the line
+workers store in raw_data (0 = never escaped).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdzg.so")
+
+MANDELBROT, BURNING_SHIP, GENERALIZED_CELTIC, VARIANT = 0, 1, 2, 3
+FAMILY_MANDEL, FAMILY_JULIA = 0, 1
+PATH_MPFR, PATH_LD, PATH_DOUBLE = 0, 1, 2
+BATCH_PERSISTENT, BATCH_PER_TILE = 0, 1
+MAX_PRECISION = 1024
+
+ERRORS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "HIP device error",
+          -4: "render stopped", -5: "admission refused", -6: "coordinate string did not parse",
+          -7: "precision or size out of range", -8: "call out of sequence"}
+
+
+class MdzgError(RuntimeError):
+    def __init__(self, code, what=""):
+        super().__init__(f"{what}: {ERRORS.get(code, code)} ({code})")
+        self.code = code
+
+
+class View(ctypes.Structure):
+    _fields_ = [
+        ("fractal", ctypes.c_int), ("family", ctypes.c_int), ("path", ctypes.c_int),
+        ("real_width", ctypes.c_int), ("real_height", ctypes.c_int),
+        ("precision", ctypes.c_long), ("depth", ctypes.c_longlong),
+        ("xmin", ctypes.c_char_p), ("xmax", ctypes.c_char_p),
+        ("ymax", ctypes.c_char_p), ("width", ctypes.c_char_p),
+        ("julia_re", ctypes.c_char_p), ("julia_im", ctypes.c_char_p),
+    ]
+
+
+class Tile(ctypes.Structure):
+    _fields_ = [("view", View), ("line0", ctypes.c_int), ("nlines", ctypes.c_int),
+                ("col0", ctypes.c_int), ("ncols", ctypes.c_int),
+                ("counts", ctypes.POINTER(ctypes.c_uint32))]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
+                ("pixel_iters", ctypes.c_double), ("launches", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libmdzg.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m mdz_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        vp = ctypes.c_void_p
+        L.mdzg_abi_version.restype = ctypes.c_int
+        L.mdzg_strerror.argtypes = [ctypes.c_int]
+        L.mdzg_strerror.restype = ctypes.c_char_p
+        L.mdzg_create.argtypes = [ctypes.c_int, P(vp)]
+        L.mdzg_destroy.argtypes = [vp]
+        L.mdzg_destroy.restype = None
+        L.mdzg_rect_from_center.argtypes = [ctypes.c_char_p] * 3 + [
+            ctypes.c_long, ctypes.c_int, ctypes.c_int] + [ctypes.c_char_p] * 4 + [ctypes.c_int]
+        L.mdzg_render_tile.argtypes = [vp, P(Tile), P(Stats)]
+        L.mdzg_render_batch.argtypes = [vp, P(Tile), ctypes.c_int, ctypes.c_int, P(Stats)]
+        L.mdzg_submit.argtypes = [vp, P(Tile), ctypes.c_int, P(vp)]
+        L.mdzg_poll.argtypes = [vp]
+        L.mdzg_wait.argtypes = [vp, P(Stats)]
+        L.mdzg_cancel.argtypes = [vp]
+        L.mdzg_job_free.argtypes = [vp]
+        L.mdzg_job_free.restype = None
+        L.mdzg_frame_open.argtypes = [vp, P(View), P(ctypes.c_int32), P(vp)]
+        L.mdzg_frame_calculate_line.argtypes = [vp, ctypes.c_int]
+        L.mdzg_frame_stop.argtypes = [vp]
+        L.mdzg_frame_stop.restype = None
+        L.mdzg_frame_close.argtypes = [vp, P(Stats)]
+        _lib = L
+    return _lib
+
+
+def _b(s):
+    if s is None:
+        return None
+    return s.encode() if isinstance(s, str) else s
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise MdzgError(rc, what)
+    return rc
+
+
+def rect_from_center(cx, cy, size, precision, user_w, user_h):
+    """img->xmin, xmax, ymax, width (hex strings at max(p,80) bits) of a centre view,
+    as the command-line path computes them (src/coords.c:204-313)."""
+    cap = 4096
+    bufs = [ctypes.create_string_buffer(cap) for _ in range(4)]
+    _check(lib().mdzg_rect_from_center(_b(cx), _b(cy), _b(size), precision, user_w, user_h,
+                                       *bufs, cap), "rect_from_center")
+    return tuple(b.value.decode() for b in bufs)
+
+
+def make_view(rect, *, precision, depth, width, height, fractal=MANDELBROT,
+              family=FAMILY_MANDEL, path=PATH_MPFR, julia=(None, None)):
+    """View for rect = (xmin, xmax, ymax, width) strings.  The returned View keeps
+    its strings alive through the `_keep` attribute."""
+    keep = [_b(s) for s in rect] + [_b(julia[0]), _b(julia[1])]
+    v = View(fractal, family, path, width, height, precision, depth, *keep)
+    v._keep = keep
+    return v
+
+
+class Context:
+    """One device context (mdzg_create)."""
+
+    def __init__(self, device=0):
+        self._c = ctypes.c_void_p()
+        _check(lib().mdzg_create(device, ctypes.byref(self._c)), "mdzg_create")
+
+    def close(self):
+        if self._c:
+            lib().mdzg_destroy(self._c)
+            self._c = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def _tile(view, lines, cols):
+        line0, nlines = lines if lines is not None else (0, view.real_height)
+        col0, ncols = cols if cols is not None else (0, view.real_width)
+        out = np.zeros((nlines, ncols), dtype=np.uint32)
+        t = Tile(view, line0, nlines, col0, ncols,
+                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+        return t, out
+
+    def render(self, view, lines=None, cols=None):
+        """Counts of a rectangle of a view: uint32 [nlines, ncols]; plus stats."""
+        t, out = self._tile(view, lines, cols)
+        st = Stats()
+        _check(lib().mdzg_render_tile(self._c, ctypes.byref(t), ctypes.byref(st)), "render")
+        return out, st.as_dict()
+
+    def render_batch(self, specs, mode=BATCH_PERSISTENT):
+        """specs: list of (view, lines, cols).  Returns ([counts...], stats)."""
+        n = len(specs)
+        arr = (Tile * n)()
+        outs = []
+        for i, (v, lines, cols) in enumerate(specs):
+            t, out = self._tile(v, lines, cols)
+            arr[i] = t
+            outs.append(out)
+        st = Stats()
+        _check(lib().mdzg_render_batch(self._c, arr, n, mode, ctypes.byref(st)), "render_batch")
+        return outs, st.as_dict()
+
+    def submit(self, specs):
+        """Asynchronous batch: returns a Job."""
+        return Job(self, specs)
+
+
+class Job:
+    def __init__(self, ctx, specs):
+        n = len(specs)
+        self._arr = (Tile * n)()
+        self.counts = []
+        for i, (v, lines, cols) in enumerate(specs):
+            t, out = Context._tile(v, lines, cols)
+            self._arr[i] = t
+            self.counts.append(out)
+        self._views = [s[0] for s in specs]
+        self._j = ctypes.c_void_p()
+        _check(lib().mdzg_submit(ctx._c, self._arr, n, ctypes.byref(self._j)), "submit")
+
+    def poll(self):
+        return _check(lib().mdzg_poll(self._j), "poll") == 1
+
+    def cancel(self):
+        _check(lib().mdzg_cancel(self._j), "cancel")
+
+    def wait(self):
+        st = Stats()
+        rc = lib().mdzg_wait(self._j, ctypes.byref(st))
+        return rc, st.as_dict()
+
+    def free(self):
+        if self._j:
+            lib().mdzg_job_free(self._j)
+            self._j = ctypes.c_void_p()
+
+
+class Frame:
+    """The line-callback adapter: calculate_line(line) fills row `line` of raw
+    (int32, real_height x real_width) and returns 1, or 0 once stopped."""
+
+    def __init__(self, ctx, view, raw):
+        assert raw.dtype == np.int32 and raw.flags.c_contiguous
+        self.view = view
+        self.raw = raw
+        self._f = ctypes.c_void_p()
+        _check(lib().mdzg_frame_open(ctx._c, ctypes.byref(view),
+                                     raw.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                     ctypes.byref(self._f)), "frame_open")
+
+    def calculate_line(self, line):
+        return lib().mdzg_frame_calculate_line(self._f, line)
+
+    def stop(self):
+        lib().mdzg_frame_stop(self._f)
+
+    def close(self):
+        st = Stats()
+        rc = lib().mdzg_frame_close(self._f, ctypes.byref(st))
+        self._f = ctypes.c_void_p()
+        return rc, st.as_dict()

@@ -1,0 +1,313 @@
+#ifndef MDZG_K_ESCAPE_H
+#define MDZG_K_ESCAPE_H
+/*
+ * k_escape.h -- gfx950 escape-time kernels (included by k_fp64.hip and, once
+ * per word count N, by k_rn.hip).
+ *
+ * Two arithmetics:
+ *   FP64 : the loop of src/frac_mandel.c:5-21 (and siblings) in IEEE double,
+ *          c from the double restatement of src/fractal.c:61-72.  Built with
+ *          -ffp-contract=off (and the pragma below) so every operation is one
+ *          IEEE rounding in C expression order, as the CPU double loop does.
+ *   RN<N>: the MPFR loop of src/frac_mandel.c:33-51 (and siblings) with every
+ *          operation rounded to nearest-even at the tile's precision p <= 32N
+ *          (mp_rn.h), z0 and c from host-computed per-column/per-row values.
+ *
+ * Two launch forms per arithmetic:
+ *   *_tile    : one thread per pixel of one tile (the one-launch-per-tile form);
+ *   *_persist : a grid sized to the resident capacity of the GPU; each wave
+ *               keeps 64 pixels in flight and refills lanes whose pixel has
+ *               finished from a device queue (one 64-bit counter over the flat
+ *               pixel space of all tiles of the class), so tiles of uneven
+ *               size, depth and formula share one launch and lanes do not idle
+ *               behind the slowest pixel of their wave.  Every wave leaves once
+ *               the queue is drained and its lanes are done, or when the host
+ *               raises the cancel flag.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mdzg_dev.h"
+#include "mp_rn.h"
+
+#pragma clang fp contract(off)
+
+using namespace mdzg;
+
+enum { F_MANDELBROT = 0, F_BURNING_SHIP = 1, F_GENERALIZED_CELTIC = 2, F_VARIANT = 3 };
+
+/* ------------------------------------------------------------------ */
+/* shared helpers                                                       */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ int find_tile(const mdzg_dtile* tiles, int ntiles, uint64_t g)
+{
+    int lo = 0, hi = ntiles - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tiles[mid].pix_begin <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+/* wave-aggregated claim of one queue slot per requesting lane */
+__device__ __forceinline__ uint64_t claim(bool need, unsigned long long* counter, bool& got_any)
+{
+    const uint64_t m = __ballot(need);
+    got_any = m != 0;
+    uint64_t g = ~0ull;
+    if (m) {
+        const int lane = __lane_id();
+        const int leader = __ffsll((long long)m) - 1;
+        unsigned long long base = 0;
+        if (lane == leader)
+            base = atomicAdd(counter, (unsigned long long)__popcll(m));
+        base = __shfl(base, leader);
+        const uint64_t below = m & ((1ull << lane) - 1ull);
+        if (need)
+            g = base + (uint64_t)__popcll(below);
+    }
+    return g;
+}
+
+#ifdef MDZG_FP64_KERNELS
+/* ------------------------------------------------------------------ */
+/* FP64                                                                 */
+/* ------------------------------------------------------------------ */
+struct fp64_state {
+    double wre, wim, wre2, wim2, cre, cim;
+};
+
+__device__ __forceinline__ void fp64_init(fp64_state& s, const mdzg_dtile& t, int row, int col)
+{
+    const int ix = t.col0 + col;
+    const int line = t.line0 + row;
+    const double W = (double)t.W;
+    const double y = t.ymax - (t.width / W) * (double)line;
+    const double x = (ix / W) * t.width + t.xmin;
+    s.wre = x;
+    s.wim = y;
+    s.wre2 = x * x;
+    s.wim2 = y * y;
+    if (t.family == 1) {
+        s.cre = t.jre;
+        s.cim = t.jim;
+    } else {
+        s.cre = x;
+        s.cim = y;
+    }
+}
+
+/* one iteration; returns true when |z|^2 > 4 (src/frac_mandel.c:13-18) */
+__device__ __forceinline__ bool fp64_step(fp64_state& s, int fractal, int32_t wz)
+{
+    if (fractal == F_BURNING_SHIP)
+        s.wim = 2.0 * fabs(s.wre * s.wim) + s.cim;
+    else
+        s.wim = 2.0 * s.wre * s.wim + s.cim;
+    if (fractal == F_GENERALIZED_CELTIC || (fractal == F_VARIANT && (wz & 1)))
+        s.wre = fabs(s.wre2 - s.wim2) + s.cre;
+    else
+        s.wre = s.wre2 - s.wim2 + s.cre;
+    s.wim2 = s.wim * s.wim;
+    s.wre2 = s.wre * s.wre;
+    return s.wim2 + s.wre2 > 4.0;
+}
+
+__global__ __launch_bounds__(256) void k_fp64_tile(const mdzg_dtile* tp)
+{
+    const mdzg_dtile t = *tp;
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = (uint32_t)t.ncols * (uint32_t)t.nrows;
+    if (idx >= npix)
+        return;
+    const int row = (int)(idx / (uint32_t)t.ncols), col = (int)(idx % (uint32_t)t.ncols);
+    fp64_state s;
+    fp64_init(s, t, row, col);
+    uint32_t count = 0;
+    for (int32_t wz = 1; wz <= t.depth; ++wz) {
+        if (fp64_step(s, t.fractal, wz)) {
+            count = (uint32_t)wz;
+            break;
+        }
+    }
+    t.out[idx] = count;
+}
+
+__global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, int ntiles,
+                                                      uint64_t total,
+                                                      unsigned long long* counter,
+                                                      const volatile int* cancel, int chunk)
+{
+    bool active = false;
+    int fractal = 0;
+    uint32_t round = 0;
+    int32_t wz = 0, depth = 0;
+    uint32_t* out = nullptr;
+    fp64_state s;
+    s.wre = s.wim = s.wre2 = s.wim2 = s.cre = s.cim = 0.0;
+    for (;;) {
+        bool any;
+        const uint64_t g = claim(!active, counter, any);
+        if (!active && g < total) {
+            const int ti = find_tile(tiles, ntiles, g);
+            const mdzg_dtile& t = tiles[ti];
+            const uint32_t local = (uint32_t)(g - t.pix_begin);
+            const int row = (int)(local / (uint32_t)t.ncols);
+            const int col = (int)(local % (uint32_t)t.ncols);
+            fp64_init(s, t, row, col);
+            fractal = t.fractal;
+            depth = t.depth;
+            out = t.out + local;
+            wz = 0;
+            active = true;
+        }
+        if (!__any(active))
+            break;
+        /* cancel flag (device memory, written by the host on another stream):
+           loaded before the chunk, tested after it, every 16th round */
+        const int stop = ((++round & 15) == 0) ? *cancel : 0;
+        for (int it = 0; it < chunk; ++it) {
+            if (active) {
+                ++wz;
+                const bool esc = fp64_step(s, fractal, wz);
+                if (esc || wz >= depth) {
+                    *out = esc ? (uint32_t)wz : 0u;
+                    active = false;
+                }
+            }
+        }
+        if (stop)
+            break;
+    }
+}
+
+#endif /* MDZG_FP64_KERNELS */
+
+/* ------------------------------------------------------------------ */
+/* RN<N>                                                                */
+/* ------------------------------------------------------------------ */
+template <int N>
+__device__ __forceinline__ void load_mp(mp<N>& v, const uint32_t* p)
+{
+#pragma unroll
+    for (int j = 0; j < N; ++j) v.m[j] = p[j];
+    v.e = (int32_t)p[N];
+    v.s = p[N + 1];
+}
+
+template <int N>
+struct rn_state {
+    mp<N> wre, wim, wre2, wim2, cre, cim;
+};
+
+template <int N, bool FULL>
+__device__ __forceinline__ void rn_init(rn_state<N>& s, const mdzg_dtile& t, int row, int col)
+{
+    const int k = FULL ? 0 : t.kclr;
+    load_mp<N>(s.wre, t.xs + (size_t)col * MDZG_VS(N));
+    load_mp<N>(s.wim, t.ys + (size_t)row * MDZG_VS(N));
+    mul_rn<N, true>(s.wre2, s.wre, s.wre, k);      /* src/fractal.c:188 */
+    mul_rn<N, true>(s.wim2, s.wim, s.wim, k);      /* src/fractal.c:171 */
+    if (t.family == 1) {
+        load_mp<N>(s.cre, t.jc);
+        load_mp<N>(s.cim, t.jc + MDZG_VS(N));
+    } else {
+        s.cre = s.wre;
+        s.cim = s.wim;
+    }
+}
+
+/* one iteration of src/frac_mandel.c:36-48 (and siblings); true on escape */
+template <int N, bool FULL>
+__device__ __forceinline__ bool rn_step(rn_state<N>& s, int fractal, int32_t wz, int kclr)
+{
+    const int k = FULL ? 0 : kclr;
+    mp<N> t;
+    mul_rn<N, false>(t, s.wre, s.wim, k);                    /* t1 = wre*wim     */
+    t.s = (fractal == F_BURNING_SHIP) ? 0u : t.s;            /* BS: |t1|         */
+    twice<N>(t);                                             /* t1 *= 2          */
+    add_rn<N>(s.wim, t, s.cim, 0u, k);                       /* wim = t1 + c_im  */
+    add_rn<N>(t, s.wre2, s.wim2, 1u, k);                     /* t1 = wre2 - wim2 */
+    const bool ab = fractal == F_GENERALIZED_CELTIC || (fractal == F_VARIANT && (wz & 1));
+    t.s = ab ? 0u : t.s;                                     /* Celtic: |t1|     */
+    add_rn<N>(s.wre, t, s.cre, 0u, k);                       /* wre = t1 + c_re  */
+    mul_rn<N, true>(s.wim2, s.wim, s.wim, k);                /* wim2 = wim^2     */
+    mul_rn<N, true>(s.wre2, s.wre, s.wre, k);                /* wre2 = wre^2     */
+    add_rn<N>(t, s.wim2, s.wre2, 0u, k);                     /* t1 = wim2 + wre2 */
+    return gt4<N>(t);                                        /* t1 > 4           */
+}
+
+template <int N, bool FULL>
+__global__ __launch_bounds__(256) void k_rn_tile(const mdzg_dtile* tp)
+{
+    const mdzg_dtile& t = *tp;
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = (uint32_t)t.ncols * (uint32_t)t.nrows;
+    if (idx >= npix)
+        return;
+    const int row = (int)(idx / (uint32_t)t.ncols), col = (int)(idx % (uint32_t)t.ncols);
+    rn_state<N> s;
+    rn_init<N, FULL>(s, t, row, col);
+    const int fractal = t.fractal, kclr = t.kclr;
+    const int32_t depth = t.depth;
+    uint32_t count = 0;
+    for (int32_t wz = 1; wz <= depth; ++wz) {
+        if (rn_step<N, FULL>(s, fractal, wz, kclr)) {
+            count = (uint32_t)wz;
+            break;
+        }
+    }
+    t.out[idx] = count;
+}
+
+template <int N, bool FULL>
+__global__ __launch_bounds__(256) void k_rn_persist(const mdzg_dtile* tiles, int ntiles,
+                                                    uint64_t total,
+                                                    unsigned long long* counter,
+                                                    const volatile int* cancel, int chunk)
+{
+    bool active = false;
+    int fractal = 0, kclr = 0;
+    uint32_t round = 0;
+    int32_t wz = 0, depth = 0;
+    uint32_t* out = nullptr;
+    rn_state<N> s;
+    for (;;) {
+        bool any;
+        const uint64_t g = claim(!active, counter, any);
+        if (!active && g < total) {
+            const int ti = find_tile(tiles, ntiles, g);
+            const mdzg_dtile& t = tiles[ti];
+            const uint32_t local = (uint32_t)(g - t.pix_begin);
+            const int row = (int)(local / (uint32_t)t.ncols);
+            const int col = (int)(local % (uint32_t)t.ncols);
+            rn_init<N, FULL>(s, t, row, col);
+            fractal = t.fractal;
+            kclr = t.kclr;
+            depth = t.depth;
+            out = t.out + local;
+            wz = 0;
+            active = true;
+        }
+        if (!__any(active))
+            break;
+        const int stop = ((++round & 15) == 0) ? *cancel : 0;
+        for (int it = 0; it < chunk; ++it) {
+            if (active) {
+                ++wz;
+                const bool esc = rn_step<N, FULL>(s, fractal, wz, kclr);
+                if (esc || wz >= depth) {
+                    *out = esc ? (uint32_t)wz : 0u;
+                    active = false;
+                }
+            }
+        }
+        if (stop)
+            break;
+    }
+}
+
+#endif

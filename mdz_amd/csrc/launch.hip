@@ -1,0 +1,83 @@
+/*
+ * launch.hip -- kernel table and launch wrappers called by the C host runtime.
+ *
+ * Kernel id: 0 = FP64; 1 + 2*i + full for the RN kernels of word count
+ * MDZG_NWORDS[i] (full = the tile precision is exactly 32N bits).
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mdzg_dev.h"
+
+extern "C" {
+void mdzg_kget_fp64(const void** tile, const void** persist);
+void mdzg_kget_rn1(int, const void**, const void**);
+void mdzg_kget_rn2(int, const void**, const void**);
+void mdzg_kget_rn3(int, const void**, const void**);
+void mdzg_kget_rn4(int, const void**, const void**);
+void mdzg_kget_rn6(int, const void**, const void**);
+void mdzg_kget_rn8(int, const void**, const void**);
+void mdzg_kget_rn12(int, const void**, const void**);
+void mdzg_kget_rn16(int, const void**, const void**);
+void mdzg_kget_rn24(int, const void**, const void**);
+void mdzg_kget_rn32(int, const void**, const void**);
+}
+
+typedef void (*kget_fn)(int, const void**, const void**);
+static const kget_fn kget_rn[MDZG_NCLASS_N] = {
+    mdzg_kget_rn1, mdzg_kget_rn2, mdzg_kget_rn3, mdzg_kget_rn4, mdzg_kget_rn6,
+    mdzg_kget_rn8, mdzg_kget_rn12, mdzg_kget_rn16, mdzg_kget_rn24, mdzg_kget_rn32,
+};
+
+static void lookup(int kid, int persist, const void** f)
+{
+    const void* t = nullptr;
+    const void* p = nullptr;
+    if (kid == 0)
+        mdzg_kget_fp64(&t, &p);
+    else
+        kget_rn[(kid - 1) >> 1]((kid - 1) & 1, &t, &p);
+    *f = persist ? p : t;
+}
+
+extern "C" {
+
+int mdzg_dev_nkernels(void) { return 1 + 2 * MDZG_NCLASS_N; }
+
+int mdzg_dev_occupancy(int kid, int persist)
+{
+    if (kid < 0 || kid >= mdzg_dev_nkernels())
+        return 0;
+    const void* f;
+    lookup(kid, persist, &f);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, 0) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+int mdzg_dev_launch_tile(int kid, const mdzg_dtile* d_tile, uint32_t npix, hipStream_t st)
+{
+    if (kid < 0 || kid >= mdzg_dev_nkernels() || npix == 0)
+        return -1;
+    const void* f;
+    lookup(kid, 0, &f);
+    void* args[] = { (void*)&d_tile };
+    return hipLaunchKernel(f, dim3((npix + 255u) / 256u), dim3(256), args, 0, st) == hipSuccess
+               ? 0 : -3;
+}
+
+int mdzg_dev_launch_persist(int kid, const mdzg_dtile* d_tiles, int ntiles, uint64_t total,
+                            unsigned long long* d_counter, const int* d_cancel, int grid,
+                            int chunk, hipStream_t st)
+{
+    if (kid < 0 || kid >= mdzg_dev_nkernels() || ntiles <= 0 || grid <= 0 || chunk <= 0)
+        return -1;
+    const void* f;
+    lookup(kid, 1, &f);
+    void* args[] = { (void*)&d_tiles, (void*)&ntiles, (void*)&total, (void*)&d_counter,
+                     (void*)&d_cancel, (void*)&chunk };
+    return hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, st) == hipSuccess ? 0 : -3;
+}
+
+} /* extern "C" */

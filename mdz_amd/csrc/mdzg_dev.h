@@ -1,0 +1,42 @@
+/*
+ * mdzg_dev.h -- layout shared by the host runtime (C) and the gfx950 kernels.
+ *
+ * A device tile is one rectangle of one view.  Pixels of all tiles of one
+ * arithmetic class form a flat index space [0, total): tile t owns
+ * [pix_begin, pix_begin + nrows*ncols), row-major inside the tile, so the
+ * lanes of a wave walk along a row (neighbouring c, similar escape times).
+ *
+ * Stored multi-word values (mp<N> of mp_rn.h) take MDZG_VS(N) 32-bit words:
+ *   w[0..N-1] significand (least significant first), w[N] exponent (int32),
+ *   w[N+1] sign, padded to a multiple of 4 words for 16-byte loads.
+ */
+#ifndef MDZG_DEV_H
+#define MDZG_DEV_H
+
+#include <stdint.h>
+
+#define MDZG_VS(N) ((((N) + 2) + 3) & ~3)
+
+/* arithmetic classes: FP64, or the RN kernels for N words (N in MDZG_NWORDS) */
+#define MDZG_NCLASS_N 10
+#define MDZG_NWORDS_INIT { 1, 2, 3, 4, 6, 8, 12, 16, 24, 32 }
+#define MDZG_CLASS_FP64 (-1)
+
+typedef struct mdzg_dtile {
+    uint32_t* out;              /* counts: nrows x ncols, row-major                  */
+    const uint32_t* xs;         /* RN: ncols stored values (z0.re / c.re per column) */
+    const uint32_t* ys;         /* RN: nrows stored values (z0.im / c.im per row)    */
+    const uint32_t* jc;         /* RN Julia: c.re, c.im (2 stored values) or 0       */
+    double xmin, ymax, width;   /* FP64: view rect, width = xmax - xmin              */
+    double jre, jim;            /* FP64 Julia c                                      */
+    uint64_t pix_begin;         /* first flat pixel index of this tile in its class  */
+    int32_t ncols, nrows;
+    int32_t col0, line0;        /* FP64: origin of the rectangle inside the view     */
+    int32_t W;                  /* FP64: view real_width (the coordinate divisor)    */
+    int32_t fractal, family;
+    int32_t kclr;               /* RN: 32N - p bits cleared by rounding              */
+    int32_t depth;              /* max iterations (image_info.depth)                 */
+    int32_t vs;                 /* RN: words per stored value                        */
+} mdzg_dtile;
+
+#endif

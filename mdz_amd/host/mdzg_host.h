@@ -1,0 +1,95 @@
+/*
+ * mdzg_host.h -- internal interfaces of the host runtime (C99).
+ */
+#ifndef MDZG_HOST_H
+#define MDZG_HOST_H
+
+#include <gmp.h>
+#include <mpfr.h>
+#include <pthread.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "../../include/mdzg.h"
+#include "../csrc/mdzg_dev.h"
+
+/* kernel table (launch.hip) */
+int mdzg_dev_nkernels(void);
+int mdzg_dev_occupancy(int kid, int persist);
+int mdzg_dev_launch_tile(int kid, const mdzg_dtile* d_tile, uint32_t npix, hipStream_t st);
+int mdzg_dev_launch_persist(int kid, const mdzg_dtile* d_tiles, int ntiles, uint64_t total,
+                            unsigned long long* d_counter, const int* d_cancel, int grid,
+                            int chunk, hipStream_t st);
+
+extern const int mdzg_nwords[MDZG_NCLASS_N];
+
+/* A parsed view: image_info fields as MPFR values at max(p, 80) bits. */
+typedef struct mdzg_pview {
+    mdzg_view v;              /* caller's scalars (string pointers not kept) */
+    long hp;                  /* max(p, 80) */
+    mpfr_t xmin, xmax, ymax, width, jre, jim;
+    int kid;                  /* kernel id */
+    int nw;                   /* words N (RN classes), 0 for FP64 */
+    int kclr;                 /* 32N - p */
+} mdzg_pview;
+
+int  mdzg_view_parse(const mdzg_view* v, mdzg_pview* pv);
+void mdzg_view_clear(mdzg_pview* pv);
+
+/* kernel id for a view (path, precision); -1 when out of range */
+int mdzg_class_of(int path, long precision, int* nw, int* kclr);
+
+/* Per-column / per-row start values of a rectangle, in stored-value format
+   (MDZG_VS(nw) words each): xs[ncols], ys[nlines], and jc[2] for Julia. */
+int mdzg_coords_fill(const mdzg_pview* pv, int line0, int nlines, int col0, int ncols,
+                     uint32_t* xs, uint32_t* ys, uint32_t* jc);
+
+/* FP64 descriptor fields (xmin, ymax, width, Julia c) */
+void mdzg_coords_fp64(const mdzg_pview* pv, double* xmin, double* ymax, double* width,
+                      double* jre, double* jim);
+
+/* mpfr (precision <= 32*nw) -> stored value */
+void mdzg_store_mpfr(uint32_t* w, int nw, mpfr_srcptr x);
+/* long double (x87, 64-bit significand) -> stored value with nw >= 2 */
+void mdzg_store_ld(uint32_t* w, int nw, long double x);
+
+/* device context */
+struct mdzg_ctx {
+    int device;
+    int ncu;
+    hipStream_t stream;         /* synchronous batches */
+    hipStream_t ctl;            /* cancel-flag writes */
+    int* d_cancel_zero;         /* a device int that stays 0 */
+    int occ[1 + 2 * MDZG_NCLASS_N];
+    pthread_mutex_t lock;       /* serialises users of the shared arena */
+    void* d_arena;
+    size_t d_cap;
+    void* h_stage;
+    size_t h_cap;
+    hipEvent_t ev0, ev1;
+};
+
+/* a staged batch on the device */
+typedef struct mdzg_plan {
+    int ntiles;
+    int nclass;                 /* classes present */
+    int cls_kid[1 + 2 * MDZG_NCLASS_N];
+    int cls_first[1 + 2 * MDZG_NCLASS_N];   /* index into the sorted tile order */
+    int cls_count[1 + 2 * MDZG_NCLASS_N];
+    uint64_t cls_total[1 + 2 * MDZG_NCLASS_N];
+    size_t off_desc, off_ctr, off_coords, off_out, bytes;
+    size_t out_bytes;
+    int* order;                 /* sorted tile order (by class) */
+    size_t* out_off;            /* per tile (original index): offset of its counts */
+    mdzg_dtile* desc;           /* host copy (in the staging buffer) */
+} mdzg_plan;
+
+int  mdzg_plan_build(mdzg_ctx* ctx, const mdzg_tile* tiles, int ntiles, mdzg_plan* pl,
+                     void** h_stage, size_t* h_cap, void* d_base_hint);
+void mdzg_plan_free(mdzg_plan* pl);
+
+double mdzg_now_ms(void);
+
+#endif

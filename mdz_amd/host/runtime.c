@@ -1,0 +1,565 @@
+/*
+ * runtime.c -- device context, staging and launch of tile batches (C99).
+ *
+ * A batch (one mdzg_job) lays out in one device allocation:
+ *   [descriptors][queue counters + cancel flag][start values][counts]
+ * staged through one pinned host buffer: a single host-to-device copy before
+ * the launches and a single device-to-host copy of the counts after them.
+ * Tiles are grouped by arithmetic class; each class gets one persistent
+ * launch draining its own queue (MDZG_BATCH_PERSISTENT), or each tile its own
+ * plain launch (MDZG_BATCH_PER_TILE, the baseline form).  Jobs run on a pool
+ * of streams so independent jobs overlap on the device.
+ *
+ * This replaces the pthread line pool of src/render_threads.c:260-393 for the
+ * device: the claim of the next piece of work (next_line += aa under a mutex,
+ * :366-369) becomes an atomic on a device counter, one pixel per lane.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "mdzg_host.h"
+
+#define NSTREAMS 8
+#define NCACHE 8
+#define ALIGN_UP(x, a) (((x) + (size_t)(a) - 1) & ~((size_t)(a) - 1))
+
+typedef struct cache_slot {
+    void* d;
+    size_t dcap;
+    void* h;
+    size_t hcap;
+    int used;
+} cache_slot;
+
+struct mdzg_rt {
+    hipStream_t streams[NSTREAMS];
+    int next_stream;
+    cache_slot cache[NCACHE];
+    pthread_mutex_t cache_lock;
+};
+
+/* the ctx struct of mdzg_host.h plus the runtime pools */
+typedef struct ctx_full {
+    struct mdzg_ctx c;
+    struct mdzg_rt rt;
+} ctx_full;
+
+struct mdzg_job {
+    mdzg_ctx* ctx;
+    int ntiles;
+    mdzg_tile* tiles;            /* copies of the caller's tile descriptors */
+    int mode;
+    hipStream_t stream;
+    int slot;                    /* cache slot index, or -1 when owned */
+    void* d;
+    void* h;
+    size_t off_ctr, off_coords, off_out, bytes, out_bytes;
+    size_t* out_off;
+    int nkid;
+    int kids[1 + 2 * MDZG_NCLASS_N];
+    hipEvent_t ev0, ev1;
+    double t_start;
+    int launched;
+    int done;
+    int rc;
+    int launches;
+    int cancelled;
+};
+
+double mdzg_now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return 1e3 * (double)ts.tv_sec + 1e-6 * (double)ts.tv_nsec;
+}
+
+int mdzg_abi_version(void) { return MDZG_ABI_VERSION; }
+
+const char* mdzg_strerror(int code)
+{
+    switch (code) {
+    case MDZG_OK: return "ok";
+    case MDZG_ERR_ARG: return "invalid argument";
+    case MDZG_ERR_NOMEM: return "out of memory";
+    case MDZG_ERR_DEVICE: return "HIP device error";
+    case MDZG_ERR_STOPPED: return "render stopped";
+    case MDZG_ERR_BUSY: return "admission refused";
+    case MDZG_ERR_PARSE: return "coordinate string did not parse";
+    case MDZG_ERR_RANGE: return "precision or size out of range";
+    case MDZG_ERR_STATE: return "call out of sequence";
+    default: return "unknown error";
+    }
+}
+
+int mdzg_create(int device, mdzg_ctx** out)
+{
+    if (!out)
+        return MDZG_ERR_ARG;
+    *out = NULL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return MDZG_ERR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess)
+        return MDZG_ERR_DEVICE;
+    ctx_full* f = (ctx_full*)calloc(1, sizeof *f);
+    if (!f)
+        return MDZG_ERR_NOMEM;
+    mdzg_ctx* c = &f->c;
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        goto fail;
+    c->ncu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->ctl, hipStreamNonBlocking) != hipSuccess)
+        goto fail;
+    for (int i = 0; i < NSTREAMS; ++i)
+        if (hipStreamCreateWithFlags(&f->rt.streams[i], hipStreamNonBlocking) != hipSuccess)
+            goto fail;
+    if (hipMalloc((void**)&c->d_cancel_zero, 256) != hipSuccess)
+        goto fail;
+    if (hipMemset(c->d_cancel_zero, 0, 256) != hipSuccess)
+        goto fail;
+    for (int k = 0; k < mdzg_dev_nkernels(); ++k) {
+        c->occ[k] = mdzg_dev_occupancy(k, 1);
+        if (c->occ[k] < 1)
+            c->occ[k] = 1;
+    }
+    pthread_mutex_init(&c->lock, NULL);
+    pthread_mutex_init(&f->rt.cache_lock, NULL);
+    *out = c;
+    return MDZG_OK;
+fail:
+    free(f);
+    return MDZG_ERR_DEVICE;
+}
+
+void mdzg_destroy(mdzg_ctx* c)
+{
+    if (!c)
+        return;
+    ctx_full* f = (ctx_full*)c;
+    hipSetDevice(c->device);
+    hipDeviceSynchronize();
+    for (int i = 0; i < NCACHE; ++i) {
+        if (f->rt.cache[i].d) hipFree(f->rt.cache[i].d);
+        if (f->rt.cache[i].h) hipHostFree(f->rt.cache[i].h);
+    }
+    for (int i = 0; i < NSTREAMS; ++i)
+        if (f->rt.streams[i]) hipStreamDestroy(f->rt.streams[i]);
+    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->ctl) hipStreamDestroy(c->ctl);
+    if (c->d_cancel_zero) hipFree(c->d_cancel_zero);
+    pthread_mutex_destroy(&c->lock);
+    pthread_mutex_destroy(&f->rt.cache_lock);
+    free(f);
+}
+
+/* ---- buffer cache -------------------------------------------------------- */
+static int acquire(mdzg_ctx* c, size_t bytes, void** d, void** h)
+{
+    ctx_full* f = (ctx_full*)c;
+    pthread_mutex_lock(&f->rt.cache_lock);
+    int best = -1;
+    for (int i = 0; i < NCACHE; ++i) {
+        cache_slot* s = &f->rt.cache[i];
+        if (!s->used && s->d && s->dcap >= bytes && (best < 0 || s->dcap < f->rt.cache[best].dcap))
+            best = i;
+    }
+    if (best < 0) {
+        /* grow a free slot (or take an empty one) */
+        for (int i = 0; i < NCACHE; ++i) {
+            cache_slot* s = &f->rt.cache[i];
+            if (!s->used) {
+                if (s->d) { hipFree(s->d); s->d = NULL; }
+                if (s->h) { hipHostFree(s->h); s->h = NULL; }
+                size_t cap = ALIGN_UP(bytes + bytes / 4, 1 << 20);
+                if (hipMalloc(&s->d, cap) != hipSuccess) { s->d = NULL; break; }
+                if (hipHostMalloc(&s->h, cap, hipHostMallocDefault) != hipSuccess) {
+                    hipFree(s->d);
+                    s->d = NULL;
+                    s->h = NULL;
+                    break;
+                }
+                s->dcap = s->hcap = cap;
+                best = i;
+                break;
+            }
+        }
+    }
+    if (best >= 0) {
+        f->rt.cache[best].used = 1;
+        *d = f->rt.cache[best].d;
+        *h = f->rt.cache[best].h;
+    }
+    pthread_mutex_unlock(&f->rt.cache_lock);
+    return best;
+}
+
+static void release(mdzg_ctx* c, int slot)
+{
+    ctx_full* f = (ctx_full*)c;
+    pthread_mutex_lock(&f->rt.cache_lock);
+    if (slot >= 0 && slot < NCACHE)
+        f->rt.cache[slot].used = 0;
+    pthread_mutex_unlock(&f->rt.cache_lock);
+}
+
+static hipStream_t pick_stream(mdzg_ctx* c)
+{
+    ctx_full* f = (ctx_full*)c;
+    pthread_mutex_lock(&f->rt.cache_lock);
+    hipStream_t s = f->rt.streams[f->rt.next_stream];
+    f->rt.next_stream = (f->rt.next_stream + 1) % NSTREAMS;
+    pthread_mutex_unlock(&f->rt.cache_lock);
+    return s;
+}
+
+/* ---- job construction ---------------------------------------------------- */
+static int tile_ok(const mdzg_tile* t)
+{
+    return t && t->counts && t->nlines > 0 && t->ncols > 0 && t->line0 >= 0 && t->col0 >= 0 &&
+           t->line0 + t->nlines <= t->view.real_height && t->col0 + t->ncols <= t->view.real_width &&
+           (uint64_t)t->nlines * (uint64_t)t->ncols <= 0x7fffffffull;
+}
+
+static void job_free_mem(mdzg_job* j)
+{
+    if (j->ev0) hipEventDestroy(j->ev0);
+    if (j->ev1) hipEventDestroy(j->ev1);
+    if (j->slot >= 0) {
+        release(j->ctx, j->slot);
+    } else {
+        if (j->d) hipFree(j->d);
+        if (j->h) hipHostFree(j->h);
+    }
+    free(j->tiles);
+    free(j->out_off);
+    free(j);
+}
+
+static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int mode, mdzg_job** out)
+{
+    if (!c || !tiles || ntiles < 1 || !out || (mode != MDZG_BATCH_PERSISTENT && mode != MDZG_BATCH_PER_TILE))
+        return MDZG_ERR_ARG;
+    *out = NULL;
+    for (int i = 0; i < ntiles; ++i)
+        if (!tile_ok(&tiles[i]))
+            return MDZG_ERR_ARG;
+
+    mdzg_job* j = (mdzg_job*)calloc(1, sizeof *j);
+    if (!j)
+        return MDZG_ERR_NOMEM;
+    j->ctx = c;
+    j->slot = -1;
+    j->mode = mode;
+    j->ntiles = ntiles;
+    j->tiles = (mdzg_tile*)malloc(sizeof(mdzg_tile) * (size_t)ntiles);
+    j->out_off = (size_t*)malloc(sizeof(size_t) * (size_t)ntiles);
+    mdzg_pview* pv = (mdzg_pview*)calloc((size_t)ntiles, sizeof(mdzg_pview));
+    int* order = (int*)malloc(sizeof(int) * (size_t)ntiles);
+    if (!j->tiles || !j->out_off || !pv || !order) {
+        free(pv);
+        free(order);
+        job_free_mem(j);
+        return MDZG_ERR_NOMEM;
+    }
+    memcpy(j->tiles, tiles, sizeof(mdzg_tile) * (size_t)ntiles);
+    hipSetDevice(c->device);
+
+    int rc = MDZG_OK;
+    int nparsed = 0;
+    for (int i = 0; i < ntiles && rc == MDZG_OK; ++i, ++nparsed)
+        rc = mdzg_view_parse(&tiles[i].view, &pv[i]);
+    if (rc != MDZG_OK) {
+        nparsed--;   /* the failing parse cleaned up after itself */
+        goto done;
+    }
+
+    /* order tiles by kernel id (stable) */
+    {
+        int n = 0;
+        for (int k = 0; k < mdzg_dev_nkernels(); ++k) {
+            int any = 0;
+            for (int i = 0; i < ntiles; ++i)
+                if (pv[i].kid == k) {
+                    order[n++] = i;
+                    any = 1;
+                }
+            if (any)
+                j->kids[j->nkid++] = k;
+        }
+    }
+
+    /* layout */
+    size_t off = 0;
+    const size_t off_desc = 0;
+    off = ALIGN_UP(sizeof(mdzg_dtile) * (size_t)ntiles, 256);
+    j->off_ctr = off;
+    off = ALIGN_UP(off + 8 * (size_t)(j->nkid + 1), 256);   /* counters + cancel flag */
+    j->off_coords = off;
+    for (int i = 0; i < ntiles; ++i) {
+        if (pv[i].nw) {
+            size_t nval = (size_t)tiles[i].ncols + (size_t)tiles[i].nlines + 2;
+            off += ALIGN_UP(nval * 4 * (size_t)MDZG_VS(pv[i].nw), 16);
+        }
+    }
+    off = ALIGN_UP(off, 256);
+    j->off_out = off;
+    for (int i = 0; i < ntiles; ++i) {
+        j->out_off[i] = off;
+        off += ALIGN_UP((size_t)tiles[i].nlines * (size_t)tiles[i].ncols * 4, 16);
+    }
+    j->bytes = ALIGN_UP(off, 256);
+    j->out_bytes = j->bytes - j->off_out;
+
+    j->slot = acquire(c, j->bytes, &j->d, &j->h);
+    if (j->slot < 0) {
+        /* every cached buffer is in use: this job owns its memory */
+        if (hipMalloc(&j->d, j->bytes) != hipSuccess) {
+            j->d = NULL;
+            rc = MDZG_ERR_NOMEM;
+            goto done;
+        }
+        if (hipHostMalloc(&j->h, j->bytes, hipHostMallocDefault) != hipSuccess) {
+            j->h = NULL;
+            rc = MDZG_ERR_NOMEM;
+            goto done;
+        }
+    }
+
+    /* fill staging: descriptors, counters, start values */
+    {
+        char* H = (char*)j->h;
+        char* D = (char*)j->d;
+        memset(H, 0, j->off_out);
+        mdzg_dtile* desc = (mdzg_dtile*)(H + off_desc);
+        size_t coff = j->off_coords;
+        uint64_t pix_begin = 0;
+        int prev_kid = -1;
+        for (int n = 0; n < ntiles; ++n) {
+            const int i = order[n];
+            const mdzg_tile* t = &tiles[i];
+            mdzg_dtile* d = &desc[n];
+            if (pv[i].kid != prev_kid) {
+                pix_begin = 0;
+                prev_kid = pv[i].kid;
+            }
+            d->out = (uint32_t*)(D + j->out_off[i]);
+            d->ncols = t->ncols;
+            d->nrows = t->nlines;
+            d->col0 = t->col0;
+            d->line0 = t->line0;
+            d->W = t->view.real_width;
+            d->fractal = t->view.fractal;
+            d->family = t->view.family;
+            d->kclr = pv[i].kclr;
+            d->depth = (int32_t)t->view.depth;
+            d->vs = pv[i].nw ? MDZG_VS(pv[i].nw) : 0;
+            d->pix_begin = pix_begin;
+            pix_begin += (uint64_t)t->ncols * (uint64_t)t->nlines;
+            if (pv[i].nw) {
+                const size_t vsb = 4 * (size_t)MDZG_VS(pv[i].nw);
+                uint32_t* xs = (uint32_t*)(H + coff);
+                uint32_t* ys = (uint32_t*)(H + coff + vsb * (size_t)t->ncols);
+                uint32_t* jc = (uint32_t*)(H + coff + vsb * ((size_t)t->ncols + (size_t)t->nlines));
+                rc = mdzg_coords_fill(&pv[i], t->line0, t->nlines, t->col0, t->ncols, xs, ys, jc);
+                if (rc != MDZG_OK)
+                    goto done;
+                d->xs = (const uint32_t*)(D + coff);
+                d->ys = (const uint32_t*)(D + coff + vsb * (size_t)t->ncols);
+                d->jc = (const uint32_t*)(D + coff + vsb * ((size_t)t->ncols + (size_t)t->nlines));
+                coff += ALIGN_UP(((size_t)t->ncols + (size_t)t->nlines + 2) * vsb, 16);
+            } else {
+                mdzg_coords_fp64(&pv[i], &d->xmin, &d->ymax, &d->width, &d->jre, &d->jim);
+            }
+        }
+    }
+
+    /* copy in, launch, copy out */
+    {
+        char* D = (char*)j->d;
+        const mdzg_dtile* d_desc = (const mdzg_dtile*)(D + off_desc);
+        unsigned long long* d_ctr = (unsigned long long*)(D + j->off_ctr);
+        const int* d_cancel = (const int*)(D + j->off_ctr + 8 * (size_t)j->nkid);
+        j->stream = pick_stream(c);
+        if (hipEventCreate(&j->ev0) != hipSuccess || hipEventCreate(&j->ev1) != hipSuccess) {
+            rc = MDZG_ERR_DEVICE;
+            goto done;
+        }
+        if (hipMemcpyAsync(j->d, j->h, j->off_out, hipMemcpyHostToDevice, j->stream) != hipSuccess) {
+            rc = MDZG_ERR_DEVICE;
+            goto done;
+        }
+        /* counts start at the UINT_MAX sentinel of src/image_info.c:214-215, so a
+           cancelled job leaves the pixels it never finished recognisable */
+        if (hipMemsetD32Async((hipDeviceptr_t)(D + j->off_out), 0xffffffffu, j->out_bytes / 4,
+                              j->stream) != hipSuccess) {
+            rc = MDZG_ERR_DEVICE;
+            goto done;
+        }
+        hipEventRecord(j->ev0, j->stream);
+        int first = 0;
+        for (int k = 0; k < j->nkid; ++k) {
+            const int kid = j->kids[k];
+            int cnt = 0;
+            uint64_t total = 0;
+            while (first + cnt < ntiles && pv[order[first + cnt]].kid == kid) {
+                const mdzg_tile* t = &tiles[order[first + cnt]];
+                total += (uint64_t)t->ncols * (uint64_t)t->nlines;
+                ++cnt;
+            }
+            if (mode == MDZG_BATCH_PERSISTENT) {
+                uint64_t cap = (uint64_t)c->ncu * (uint64_t)c->occ[kid];
+                uint64_t need = (total + 255) / 256;
+                int grid = (int)(need < cap ? need : cap);
+                int chunk = (kid == 0) ? 32 : 1;
+                if (mdzg_dev_launch_persist(kid, d_desc + first, cnt, total, d_ctr + k, d_cancel,
+                                            grid, chunk, j->stream) != 0) {
+                    rc = MDZG_ERR_DEVICE;
+                    goto done;
+                }
+                j->launches++;
+            } else {
+                for (int n = 0; n < cnt; ++n) {
+                    const mdzg_tile* t = &tiles[order[first + n]];
+                    uint32_t npix = (uint32_t)t->ncols * (uint32_t)t->nlines;
+                    if (mdzg_dev_launch_tile(kid, d_desc + first + n, npix, j->stream) != 0) {
+                        rc = MDZG_ERR_DEVICE;
+                        goto done;
+                    }
+                    j->launches++;
+                }
+            }
+            first += cnt;
+        }
+        hipEventRecord(j->ev1, j->stream);
+        if (hipMemcpyAsync((char*)j->h + j->off_out, D + j->off_out, j->out_bytes,
+                           hipMemcpyDeviceToHost, j->stream) != hipSuccess) {
+            rc = MDZG_ERR_DEVICE;
+            goto done;
+        }
+        j->launched = 1;
+    }
+
+done:
+    for (int i = 0; i < nparsed; ++i)
+        mdzg_view_clear(&pv[i]);
+    free(pv);
+    free(order);
+    if (rc != MDZG_OK) {
+        if (j->launched || j->stream)
+            hipStreamSynchronize(j->stream);
+        job_free_mem(j);
+        return rc;
+    }
+    *out = j;
+    return MDZG_OK;
+}
+
+int mdzg_submit(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, mdzg_job** out)
+{
+    int rc = mdzg_job_start(c, tiles, ntiles, MDZG_BATCH_PERSISTENT, out);
+    if (rc == MDZG_OK)
+        (*out)->t_start = mdzg_now_ms();
+    return rc;
+}
+
+int mdzg_poll(mdzg_job* j)
+{
+    if (!j)
+        return MDZG_ERR_ARG;
+    if (j->done)
+        return 1;
+    hipError_t e = hipStreamQuery(j->stream);
+    if (e == hipSuccess)
+        return 1;
+    if (e == hipErrorNotReady)
+        return 0;
+    return MDZG_ERR_DEVICE;
+}
+
+int mdzg_cancel(mdzg_job* j)
+{
+    if (!j || !j->launched)
+        return MDZG_ERR_ARG;
+    mdzg_ctx* c = j->ctx;
+    int* d_cancel = (int*)((char*)j->d + j->off_ctr + 8 * (size_t)j->nkid);
+    hipSetDevice(c->device);
+    static const int one = 1;
+    if (hipMemcpyAsync(d_cancel, &one, sizeof one, hipMemcpyHostToDevice, c->ctl) != hipSuccess)
+        return MDZG_ERR_DEVICE;
+    hipStreamSynchronize(c->ctl);
+    j->cancelled = 1;
+    return MDZG_OK;
+}
+
+int mdzg_wait(mdzg_job* j, mdzg_stats* st)
+{
+    if (!j)
+        return MDZG_ERR_ARG;
+    if (!j->done) {
+        if (hipStreamSynchronize(j->stream) != hipSuccess)
+            return MDZG_ERR_DEVICE;
+        /* scatter counts to the callers' buffers */
+        double pi = 0.0;
+        for (int i = 0; i < j->ntiles; ++i) {
+            const mdzg_tile* t = &j->tiles[i];
+            const uint32_t* src = (const uint32_t*)((char*)j->h + j->out_off[i]);
+            size_t n = (size_t)t->nlines * (size_t)t->ncols;
+            const double depth = (double)t->view.depth;
+            if (!j->cancelled) {
+                memcpy(t->counts, src, n * 4);
+                for (size_t k = 0; k < n; ++k)
+                    pi += src[k] ? (double)src[k] : depth;
+            } else {
+                /* keep the caller's contents where the device never wrote */
+                for (size_t k = 0; k < n; ++k)
+                    if (src[k] != 0xffffffffu) {
+                        t->counts[k] = src[k];
+                        pi += src[k] ? (double)src[k] : depth;
+                    }
+            }
+        }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, j->ev0, j->ev1);
+        j->done = 1;
+        j->rc = j->cancelled ? MDZG_ERR_STOPPED : MDZG_OK;
+        if (st) {
+            st->kernel_ms = ms;
+            st->wall_ms = mdzg_now_ms() - j->t_start;
+            st->pixel_iters = pi;
+            st->launches = j->launches;
+        }
+    }
+    return j->rc;
+}
+
+void mdzg_job_free(mdzg_job* j)
+{
+    if (!j)
+        return;
+    if (j->launched && !j->done)
+        hipStreamSynchronize(j->stream);
+    job_free_mem(j);
+}
+
+int mdzg_render_batch(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int mode, mdzg_stats* st)
+{
+    double t0 = mdzg_now_ms();
+    mdzg_job* j = NULL;
+    int rc = mdzg_job_start(c, tiles, ntiles, mode, &j);
+    if (rc != MDZG_OK)
+        return rc;
+    j->t_start = t0;
+    rc = mdzg_wait(j, st);
+    mdzg_job_free(j);
+    return rc;
+}
+
+int mdzg_render_tile(mdzg_ctx* c, const mdzg_tile* tile, mdzg_stats* st)
+{
+    return mdzg_render_batch(c, tile, 1, MDZG_BATCH_PERSISTENT, st);
+}
