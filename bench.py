@@ -1,0 +1,166 @@
+"""Benchmark: G pixel-iterations/s of the escape-time path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2|1|3|4|5]
+                    [--no-cpu-baseline]
+
+A step is one pass of the hot path over one batch of synthetic input.  The
+default workload is config 2 (the configuration the metric is quoted on):
+Mandelbrot on the MPFR path at p = 128, 640x480 at the seahorse-valley centre,
+width 1e-25, depth 5000; every pixel is inside, so a step is exactly
+640*480*5000 = 1.536e9 pixel-iterations.  The line also carries the FP64
+figure (config 1, 240x180, depth 1000), the 1024-bit figure (a band of
+config 5 rows) and, with --config 3/4, the batched and serving forms.
+
+pixel-iterations = sum over samples of (count if count > 0 else depth): the
+iterations the reference loop executes.  Algorithmic operations per
+pixel-iteration: 7 FP64 operations (double path), 2n^2+n 32x32-bit
+multiply-accumulates with n = p/32 (RN paths).  The roofline peak is MI355X's
+78.6 TFLOP/s FP64 vector rate; a 32x32->64 multiply-accumulate
+(v_mad_u64_u32) issues at the same rate as an FP64 FMA on gfx950 and is
+counted as 2 operations like the FMA.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "G pixel-iter/s at FP64/128/1024-bit vs MPFR CPU; p50/p99 tile latency"
+PEAK_TOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    """Core count as MDZ reads it (src/main.c:243-273: cpuN lines of /proc/stat)."""
+    n = 0
+    try:
+        with open("/proc/stat") as f:
+            for line in f:
+                if line.startswith("cpu") and line[3:4].isdigit():
+                    n += 1
+    except OSError:
+        pass
+    n = n or (os.cpu_count() or 1)
+    # the GPU box gives one GPU's job a 16-core share
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return min(n, cap) if cap > 0 else n
+
+
+def cpu_baseline(rect, precision, depth, W, H, line_step, threads):
+    """MDZ's MPFR loop on the host cores, timed on a sample of rows."""
+    from oracle import oracle as O
+    kind = "reference"
+    try:
+        O.use_reference(True)          # src/frac_*.c compiled as-is (-O0 -ggdb)
+    except Exception:
+        kind = "port"
+        O.use_reference(False)
+    nl = (H + line_step - 1) // line_step
+    counts, el = O.render(rect, precision=precision, depth=depth, width=W, height=H,
+                          lines=(0, nl, line_step), threads=threads, timed=True)
+    O.use_reference(False)
+    pi = float(((counts > 0) * counts + (counts == 0) * depth).sum())
+    return dict(value=pi / el / 1e9, unit="G pixel-iter/s", cores=threads, kind=kind,
+                sample=f"config 2 rows 0,{line_step},..({nl} rows x {W} px, {pi:.3e} "
+                       f"pixel-iterations) in {el:.2f} s, {threads} threads claiming rows")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="2")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the escape-time tiles shard across GPUs with no "
+              "exchange, and this build runs on one device only (use --gpus 1).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import mdz_amd as M
+    from mdz_amd import workloads as WL
+
+    ctx = M.Context(0)
+    res = dict(metric=METRIC, unit="G pixel-iter/s", n_gpus=1, higher_is_better=True,
+               scaling="weak", vs_baseline=None, data="synthetic")
+
+    if a.config == "2":
+        v = WL.config2_view()
+        log("config 2: warmup")
+        for _ in range(a.warmup):
+            ctx.render(v)
+        kms, pis = [], []
+        t0 = time.perf_counter()
+        for i in range(a.steps):
+            _, st = ctx.render(v)
+            kms.append(st["kernel_ms"])
+            pis.append(st["pixel_iters"])
+        el = time.perf_counter() - t0
+        log(f"config 2: {a.steps} steps in {el:.3f} s")
+        pi_step = pis[0]
+        kavg = sum(kms) / len(kms)
+        ops = WL.ops_per_pixel_iter(M.PATH_MPFR, 128)
+        achieved = 2.0 * ops * pi_step / (kavg * 1e-3) / 1e12
+        res.update(value=sum(pis) / el / 1e9, steps=a.steps, warmup=a.warmup,
+                   ms_per_step=1e3 * el / a.steps, dtype="u32 limbs, RN p=128",
+                   config=dict(workload="config2 Mandelbrot MPFR-path p=128 640x480 "
+                               "centre (-0.743643887037158704752191506114774, "
+                               "0.131825904205311970493132056385139) width 1e-25 depth 5000",
+                               pixel_iters_per_step=pi_step, kernel="k_rn_persist<4,true>"),
+                   kernel_gpix_per_s=pi_step / (kavg * 1e-3) / 1e9,
+                   roofline=dict(bound="valu", achieved=achieved, peak=PEAK_TOPS, unit="TOP/s",
+                                 frac=achieved / PEAK_TOPS, traffic=None,
+                                 kernel_ms=kavg,
+                                 ops="36 MAC (x2 ops) per pixel-iteration x 1.536e9 per launch"))
+
+        # FP64 figure: config 1 frame
+        v1 = WL.config1_view()
+        for _ in range(3):
+            ctx.render(v1)
+        k1, w1, p1 = [], [], 0
+        for _ in range(50):
+            _, st = ctx.render(v1)
+            k1.append(st["kernel_ms"])
+            w1.append(st["wall_ms"])
+            p1 = st["pixel_iters"]
+        k1a = sum(k1) / len(k1)
+        res["fp64"] = dict(workload="config1 240x180 depth 1000", pixel_iters=p1,
+                           kernel_ms=k1a, wall_ms_p50=float(np.median(w1)),
+                           kernel_gpix_per_s=p1 / (k1a * 1e-3) / 1e9,
+                           frac_fp64_peak=7 * p1 / (k1a * 1e-3) / 1e12 / PEAK_TOPS)
+        log(f"fp64: {res['fp64']['kernel_gpix_per_s']:.1f} G pixel-iter/s (kernel)")
+
+        # 1024-bit figure: a band of config 5 rows
+        v5 = WL.config5_view()
+        _, st5 = ctx.render(v5, lines=(1048, 2), cols=(0, 3840))
+        _, st5 = ctx.render(v5, lines=(1048, 16), cols=(0, 3840))
+        res["rn1024"] = dict(workload="config5 rows 1048-1063 x 3840 samples, p=1024, depth 20000",
+                             pixel_iters=st5["pixel_iters"], kernel_ms=st5["kernel_ms"],
+                             kernel_gpix_per_s=st5["pixel_iters"] / (st5["kernel_ms"] * 1e-3) / 1e9,
+                             frac_peak=2 * 2080 * st5["pixel_iters"] / (st5["kernel_ms"] * 1e-3)
+                             / 1e12 / PEAK_TOPS)
+        log(f"rn1024: {res['rn1024']['kernel_gpix_per_s']:.3f} G pixel-iter/s (kernel)")
+
+        if not a.no_cpu_baseline:
+            log("cpu baseline (MDZ MPFR loop on host cores) ...")
+            rect = [s.decode() for s in v._keep[:4]]
+            res["cpu_baseline"] = cpu_baseline(rect, 128, 5000, 640, 480, 4, host_cores())
+            log(f"cpu baseline: {res['cpu_baseline']['value']:.4f} G pixel-iter/s")
+    else:
+        raise SystemExit(f"unknown --config {a.config}")
+
+    ctx.close()
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -345,6 +345,11 @@ static void* worker(void* arg)
     return NULL;
 }
 
+static void put_hex(char* dst, int cap, mpfr_t v)
+{
+    mpfr_snprintf(dst, (size_t)cap, "%Ra", v);
+}
+
 static int set_str(mpfr_t v, const char* s)
 {
     return (s && mpfr_set_str(v, s, 0, MPFR_RNDN) == 0) ? 0 : -1;
@@ -396,11 +401,6 @@ int orc_render(const orc_frame* f, int nthreads, int32_t* out, double* elapsed_s
     mpfr_clears(J.xmin, J.xmax, J.ymax, J.width, J.jre, J.jim, (mpfr_ptr)0);
     mpfr_free_cache();
     return 0;
-}
-
-static void put_hex(char* dst, int cap, mpfr_t v)
-{
-    mpfr_snprintf(dst, (size_t)cap, "%Ra", v);
 }
 
 /* coords.c:204-231 (precision), coords.c:265-313 (center_to_rect, get_rect) */
@@ -494,5 +494,75 @@ int orc_rn_op(int op, long precision, const char* a_s, const char* b_s, char* ou
         rc = 0;
     }
     mpfr_clears(a, b, r, (mpfr_ptr)0);
+    return rc;
+}
+
+long long orc_formula_ld(int fractal, long long depth,
+                         const char* wim, const char* wre,
+                         const char* c_im, const char* c_re,
+                         const char* wim2, const char* wre2)
+{
+    if (fractal < 0 || fractal > 3 || !wim || !wre || !c_im || !c_re || !wim2 || !wre2)
+        return -1;
+    /* strtold reads decimal and hex-float strings exactly into the x87 format */
+    return run_ld(fractal, depth, strtold(wim, NULL), strtold(wre, NULL),
+                  strtold(c_im, NULL), strtold(c_re, NULL),
+                  strtold(wim2, NULL), strtold(wre2, NULL));
+}
+
+long long orc_formula_d(int fractal, long long depth, double wim, double wre,
+                        double c_im, double c_re, double wim2, double wre2)
+{
+    if (fractal < 0 || fractal > 3)
+        return -1;
+    return d_loop(fractal, depth, wim, wre, c_im, c_re, wim2, wre2);
+}
+
+int orc_pixel_c(const orc_frame* f, int line, int ix, char* xo, char* yo, int cap)
+{
+    if (!f || line < 0 || ix < 0 || ix >= f->real_width || f->real_width < 1)
+        return -1;
+    long hp = f->precision < 80 ? 80 : f->precision;
+    mpfr_t xmin, xmax, ymax, width;
+    mpfr_inits2(hp, xmin, xmax, ymax, width, (mpfr_ptr)0);
+    int rc = -1;
+    if (set_str(xmin, f->xmin) | set_str(xmax, f->xmax) | set_str(ymax, f->ymax) |
+        set_str(width, f->width))
+        goto out;
+    const int W = f->real_width;
+    if (f->path == ORC_PATH_MPFR) {
+        mpfr_prec_t p = f->precision;
+        mpfr_t t1, x, y, rw, xm, wd;
+        mpfr_inits2(p, t1, x, y, rw, xm, wd, (mpfr_ptr)0);
+        mpfr_set_si(rw, W, MPFR_RNDN);
+        mpfr_set(xm, xmin, MPFR_RNDN);
+        mpfr_set(wd, width, MPFR_RNDN);
+        mpfr_div(t1, wd, rw, MPFR_RNDN);
+        mpfr_mul_si(t1, t1, line, MPFR_RNDN);
+        mpfr_sub(y, ymax, t1, MPFR_RNDN);
+        mpfr_si_div(t1, ix, rw, MPFR_RNDN);
+        mpfr_mul(x, t1, wd, MPFR_RNDN);
+        mpfr_add(x, x, xm, MPFR_RNDN);
+        put_hex(xo, cap, x);
+        put_hex(yo, cap, y);
+        mpfr_clears(t1, x, y, rw, xm, wd, (mpfr_ptr)0);
+    } else if (f->path == ORC_PATH_LD) {
+        long double x0 = mpfr_get_ld(xmin, MPFR_RNDN), x1 = mpfr_get_ld(xmax, MPFR_RNDN);
+        long double y1 = mpfr_get_ld(ymax, MPFR_RNDN), w = x1 - x0;
+        long double y = y1 - (w / (long double)W) * (long double)line;
+        long double x = (ix / (long double)W) * w + x0;
+        snprintf(xo, (size_t)cap, "%La", x);
+        snprintf(yo, (size_t)cap, "%La", y);
+    } else {
+        double x0 = mpfr_get_d(xmin, MPFR_RNDN), x1 = mpfr_get_d(xmax, MPFR_RNDN);
+        double y1 = mpfr_get_d(ymax, MPFR_RNDN), w = x1 - x0;
+        double y = y1 - (w / (double)W) * (double)line;
+        double x = (ix / (double)W) * w + x0;
+        snprintf(xo, (size_t)cap, "%a", x);
+        snprintf(yo, (size_t)cap, "%a", y);
+    }
+    rc = 0;
+out:
+    mpfr_clears(xmin, xmax, ymax, width, (mpfr_ptr)0);
     return rc;
 }

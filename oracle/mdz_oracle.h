@@ -82,6 +82,21 @@ long long orc_formula_mpfr(int fractal, long long depth, long precision,
                            const char* wim2, const char* wre2,
                            char* out_wre, char* out_wim, int cap);
 
+/* The long double loop on explicit start values (x87, 64-bit significand),
+   given as strings that strtold converts exactly (hex-float or decimal). */
+long long orc_formula_ld(int fractal, long long depth,
+                         const char* wim, const char* wre,
+                         const char* c_im, const char* c_re,
+                         const char* wim2, const char* wre2);
+
+/* The double loop on explicit start values. */
+long long orc_formula_d(int fractal, long long depth, double wim, double wre,
+                        double c_im, double c_re, double wim2, double wre2);
+
+/* c of pixel (line, ix) on the frame's path, as the line worker computes it:
+   MPFR path hex strings at p bits; long double / double paths as "%La" / "%a". */
+int orc_pixel_c(const orc_frame* f, int line, int ix, char* x, char* y, int cap);
+
 /* Single RN operations at precision p on string operands (hex-float output),
    for per-operation checks: op 0 add, 1 sub, 2 mul. */
 int  orc_rn_op(int op, long precision, const char* a, const char* b, char* out, int cap);

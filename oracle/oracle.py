@@ -56,6 +56,13 @@ def lib():
         L.orc_formula_mpfr.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_long] + \
             [ctypes.c_char_p] * 6 + [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.orc_formula_mpfr.restype = ctypes.c_longlong
+        L.orc_formula_ld.argtypes = [ctypes.c_int, ctypes.c_longlong] + [ctypes.c_char_p] * 6
+        L.orc_formula_ld.restype = ctypes.c_longlong
+        L.orc_formula_d.argtypes = [ctypes.c_int, ctypes.c_longlong] + [ctypes.c_double] * 6
+        L.orc_formula_d.restype = ctypes.c_longlong
+        L.orc_pixel_c.argtypes = [ctypes.POINTER(Frame), ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        L.orc_pixel_c.restype = ctypes.c_int
         L.orc_rn_op.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_char_p,
                                 ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.orc_rn_op.restype = ctypes.c_int
@@ -128,3 +135,26 @@ def rn_op(op, precision, a, b):
     if lib().orc_rn_op(op, precision, _b(a), _b(b), buf, 8192) != 0:
         raise ValueError("bad operands")
     return buf.value.decode()
+
+
+def formula_ld(fractal, depth, wim, wre, c_im, c_re, wim2, wre2):
+    """The x87 long double loop on explicit start values, given as strings
+    (hex-float or decimal) that the C side converts exactly with strtold."""
+    return lib().orc_formula_ld(fractal, depth, *[_b(v) for v in
+                                                  (wim, wre, c_im, c_re, wim2, wre2)])
+
+
+def formula_d(fractal, depth, wim, wre, c_im, c_re, wim2, wre2):
+    return lib().orc_formula_d(fractal, depth, wim, wre, c_im, c_re, wim2, wre2)
+
+
+def pixel_c(rect, *, precision, width, height, line, ix, path=PATH_MPFR):
+    """(x, y) of one pixel as the line worker computes it (strings)."""
+    keep = [_b(s) for s in rect]
+    fr = Frame(0, 0, path, width, height, precision, 1, keep[0], keep[1], keep[2], keep[3],
+               None, None, 0, 0, 1, 0, 0)
+    a = ctypes.create_string_buffer(4096)
+    b = ctypes.create_string_buffer(4096)
+    if lib().orc_pixel_c(ctypes.byref(fr), line, ix, a, b, 4096) != 0:
+        raise ValueError("bad pixel")
+    return a.value.decode(), b.value.decode()

@@ -1,0 +1,209 @@
+"""GPU parity: the HIP kernels, called through the C ABI, against the oracle
+and the golden maps made from MDZ's own loops.  Tolerance: none (uint32
+count maps must be bit-identical)."""
+import random
+import threading
+
+import numpy as np
+import pytest
+
+import mdz_amd as M
+from mdz_amd import workloads as WL
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _eq(a, b):
+    return np.array_equal(np.asarray(a).astype(np.int64), np.asarray(b).astype(np.int64))
+
+
+def _rect(v):
+    return [s.decode() for s in v._keep[:4]]
+
+
+@pytest.mark.parametrize("path", [M.PATH_DOUBLE, M.PATH_LD])
+@pytest.mark.parametrize("fractal", [0, 1, 2, 3])
+def test_config1_full_frame(ctx, oracle, path, fractal):
+    v = WL.config1_view(fractal=fractal, path=path)
+    got, st = ctx.render(v)
+    ref = oracle.render(_rect(v), precision=64, depth=1000, width=240, height=180,
+                        fractal=fractal, path=path, threads=16)
+    assert _eq(got, ref)
+    assert st["pixel_iters"] == float(np.where(ref > 0, ref, 1000).sum())
+
+
+def test_golden_maps(ctx, golden):
+    """Every committed map (MDZ's own loops) reproduced on the device."""
+    for meta, expect in golden:
+        vd = meta["view"]
+        v = M.make_view(vd["rect"], precision=vd["precision"], depth=vd["depth"],
+                        width=vd["width"], height=vd["height"], fractal=vd["fractal"],
+                        family=vd["family"], path=vd["path"], julia=tuple(vd["julia"]))
+        line0, nl, step = meta["lines"]
+        col0, nc = meta["cols"]
+        if step == 1:
+            got, _ = ctx.render(v, lines=(line0, nl), cols=(col0, nc))
+        else:
+            full, _ = ctx.render(v, lines=(line0, (nl - 1) * step + 1), cols=(col0, nc))
+            got = full[::step]
+        print(meta["name"], flush=True)
+        assert _eq(got, expect), meta["name"]
+
+
+@pytest.mark.parametrize("name", sorted(cases.SCENES))
+def test_gallery_scenes(ctx, oracle, name):
+    rect, p, depth, fractal, path, (W, H) = cases.scene_rect(name)
+    lines, cols = (H // 4, min(12, H // 2)), (W // 4, 40)
+    v = M.make_view(rect, precision=p, depth=depth, width=W, height=H, fractal=fractal, path=path)
+    got, _ = ctx.render(v, lines=lines, cols=cols)
+    ref = oracle.render(rect, precision=p, depth=depth, width=W, height=H, fractal=fractal,
+                        path=path, lines=(lines[0], lines[1], 1), cols=cols, threads=16)
+    assert _eq(got, ref)
+
+
+@pytest.mark.parametrize("p", [2, 17, 32, 33, 53, 63, 64, 65, 80, 96, 100, 127, 128, 129,
+                               176, 184, 192, 255, 256, 320, 384, 500, 512, 640, 768, 1000, 1024])
+def test_every_precision_class(ctx, oracle, p):
+    """Word-aligned and ragged precisions of every word count, on a view with
+    escape-time structure (the 'test' scene's centre at a shallower zoom)."""
+    rect = oracle.rect_from_center("-1.7495931476527795538581440", "5.79069164258e-8",
+                                   "3e-9", p, 96, 72)
+    for fractal in (0, 2):
+        v = M.make_view(rect, precision=p, depth=400, width=96, height=72, fractal=fractal)
+        got, _ = ctx.render(v, lines=(30, 6), cols=(0, 96))
+        ref = oracle.render(rect, precision=p, depth=400, width=96, height=72, fractal=fractal,
+                            lines=(30, 6, 1), threads=16)
+        assert _eq(got, ref), (p, fractal)
+
+
+@pytest.mark.parametrize("fractal", [0, 1, 2, 3])
+def test_julia_family(ctx, oracle, fractal):
+    J = cases.JULIA
+    for path, p in ((M.PATH_MPFR, 96), (M.PATH_MPFR, 200), (M.PATH_LD, 64), (M.PATH_DOUBLE, 64)):
+        v = M.make_view(J["rect"], precision=p, depth=J["depth"], width=96, height=72,
+                        fractal=fractal, family=M.FAMILY_JULIA, path=path, julia=J["c"])
+        got, _ = ctx.render(v)
+        ref = oracle.render(J["rect"], precision=p, depth=J["depth"], width=96, height=72,
+                            fractal=fractal, family=M.FAMILY_JULIA, path=path, julia=J["c"],
+                            threads=16)
+        assert _eq(got, ref), (path, p)
+
+
+def test_config2_full_frame(ctx, oracle):
+    v = WL.config2_view()
+    got, st = ctx.render(v)
+    assert st["pixel_iters"] == 640 * 480 * 5000.0     # every pixel runs to depth
+    ref = oracle.render(_rect(v), precision=128, depth=5000, width=640, height=480,
+                        lines=(0, 3, 239), threads=16)
+    assert _eq(got[::239], ref)
+
+
+def test_config5_crop(ctx, oracle):
+    v = WL.config5_view()
+    got, _ = ctx.render(v, lines=(1000, 4), cols=(2000, 8))
+    ref = oracle.render(_rect(v), precision=1024, depth=20000, width=3840, height=2160,
+                        lines=(1000, 4, 1), cols=(2000, 8), threads=16)
+    assert _eq(got, ref)
+
+
+def test_batch_modes_agree_with_oracle(ctx, oracle):
+    """Mixed tiles (precision, depth, formula, size) in one persistent launch per
+    class, and one launch per tile: identical counts, equal to the oracle."""
+    rng = random.Random(3)
+    specs, refs = [], []
+    for name in ("test", "subdudehue", "satellite", "honeytrace", "spiralblues", "confirm"):
+        rect, p, depth, fractal, path, (W, H) = cases.scene_rect(name)
+        nl, nc = rng.randint(3, 9), rng.randint(5, 40)
+        l0, c0 = rng.randrange(H - nl), rng.randrange(W - nc)
+        v = M.make_view(rect, precision=p, depth=min(depth, 4000), width=W, height=H,
+                        fractal=fractal, path=path)
+        specs.append((v, (l0, nl), (c0, nc)))
+        refs.append(oracle.render(rect, precision=p, depth=min(depth, 4000), width=W, height=H,
+                                  fractal=fractal, path=path, lines=(l0, nl, 1), cols=(c0, nc),
+                                  threads=16))
+    v1 = WL.config1_view(fractal=1)
+    specs.append((v1, (10, 20), (0, 240)))
+    refs.append(oracle.render(_rect(v1), precision=64, depth=1000, width=240, height=180,
+                              fractal=1, path=2, lines=(10, 20, 1), threads=16))
+    a, sa = ctx.render_batch(specs, M.BATCH_PERSISTENT)
+    b, sb = ctx.render_batch(specs, M.BATCH_PER_TILE)
+    assert sb["launches"] == len(specs) and sa["launches"] <= len(specs)
+    for x, y, r in zip(a, b, refs):
+        assert _eq(x, y) and _eq(x, r)
+
+
+def test_tile_split_invariance(ctx):
+    """A rectangle rendered whole equals the same rectangle rendered as four tiles."""
+    rect, p, depth, fractal, path, (W, H) = cases.scene_rect("deep_embedded_julia")
+    v = M.make_view(rect, precision=p, depth=3000, width=W, height=H)
+    whole, _ = ctx.render(v, lines=(40, 40), cols=(60, 64))
+    parts, _ = ctx.render_batch([(v, (40, 20), (60, 32)), (v, (40, 20), (92, 32)),
+                                 (v, (60, 20), (60, 32)), (v, (60, 20), (92, 32))])
+    top = np.hstack([parts[0], parts[1]])
+    bot = np.hstack([parts[2], parts[3]])
+    assert _eq(whole, np.vstack([top, bot]))
+
+
+def test_frame_adapter_threads(ctx):
+    """next_line_cb shape: worker threads claim rows; result equals a batch render."""
+    rect, p, depth, fractal, path, (W, H) = cases.scene_rect("subdudehue")
+    v = M.make_view(rect, precision=p, depth=2000, width=W, height=H)
+    raw = np.full((H, W), -1, dtype=np.int32)
+    fr = M.Frame(ctx, v, raw)
+    nxt = [0]
+    lock = threading.Lock()
+    rets = []
+
+    def worker():
+        while True:
+            with lock:
+                line = nxt[0]
+                nxt[0] += 1
+            if line >= H:
+                return
+            rets.append(fr.calculate_line(line))
+
+    ts = [threading.Thread(target=worker) for _ in range(8)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    rc, st = fr.close()
+    assert rc == 0 and all(r == 1 for r in rets)
+    whole, _ = ctx.render(v)
+    assert _eq(raw, whole)
+
+
+def test_frame_stop_keeps_sentinel(ctx):
+    """A stop before the frame is done: later calls return 0 and rows never
+    rendered keep the caller's sentinel (src/image_info.c:214-215)."""
+    v = WL.config2_view()          # ~1.5e9 pixel-iterations, several bands
+    big = M.make_view(_rect(v), precision=128, depth=5000, width=640, height=2048)
+    raw = np.full((2048, 640), -1, dtype=np.int32)
+    fr = M.Frame(ctx, big, raw)
+    assert fr.calculate_line(0) == 1
+    fr.stop()
+    assert fr.calculate_line(2047) == 0
+    rc, _ = fr.close()
+    assert rc == -4
+    assert (raw[0] == 0).all()
+    assert (raw[-1] == -1).all()
+
+
+def test_async_submit_and_cancel(ctx):
+    v = WL.config2_view()
+    job = ctx.submit([(v, (0, 480), (0, 640))])
+    job.cancel()
+    rc, st = job.wait()
+    job.free()
+    assert rc in (0, -4)
+    c = job.counts[0]
+    # finished pixels are exact (0 = never escaped); unfinished keep their zero init
+    assert ((c == 0)).all()
+    job2 = ctx.submit([(WL.config1_view(), None, None)])
+    while not job2.poll():
+        pass
+    rc2, st2 = job2.wait()
+    job2.free()
+    assert rc2 == 0 and st2["pixel_iters"] > 0

@@ -1,0 +1,196 @@
+"""Pinning of the oracle (CPU only).
+
+Each oracle function is checked against something it did not compute itself:
+  * the MPFR loops and single operations: exact big-integer RN arithmetic
+    (tests/rn_ref.py) and the compiled reference loops (oracle/_ref);
+  * the long double loops: the compiled reference loops, and RN at 64 bits;
+  * the double loop: the same loop run on Python floats (IEEE double);
+  * the c chain and the rect chain: exact RN arithmetic;
+  * closed forms of the Mandelbrot iteration (fixed points, 2-cycles,
+    the |z|^2 > 4 boundary itself);
+  * the committed golden count maps made from MDZ's own loops.
+"""
+import os
+import random
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from tests import cases, rn_ref
+
+REF_MISSING = not os.path.exists(
+    os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                 "oracle", "_ref", "libfrac_ref.so"))
+
+
+def _rand_operand(rng, p):
+    m = rng.getrandbits(p) | (1 << (p - 1))
+    e = rng.randint(-40, 8)
+    v = Fraction(m) * Fraction(2) ** (e - p)
+    return -v if rng.random() < 0.5 else v
+
+
+def _hex(v):
+    return cases.hexs(v)
+
+
+@pytest.mark.parametrize("p", [2, 3, 5, 24, 53, 64, 80, 113, 128, 176, 256, 1024])
+def test_rn_ops_match_exact_rounding(oracle, p):
+    rng = random.Random(p)
+    for _ in range(150):
+        a = _rand_operand(rng, rng.choice([p, 2 * p, p + 7]))
+        b = _rand_operand(rng, rng.choice([p, 2 * p]))
+        if rng.random() < 0.25:                  # cancellation
+            b = -a + _rand_operand(rng, p) * Fraction(2) ** -rng.randint(p, p + 40)
+        for op, exact in ((0, a + b), (1, a - b), (2, a * b)):
+            got = rn_ref.parse(oracle.rn_op(op, p, _hex(a), _hex(b)))
+            assert got == rn_ref.rn(exact, p), (p, op)
+    # ties: exact midpoints round to even
+    one = Fraction(1)
+    ulp = Fraction(2) ** (1 - p)
+    for k in range(4):
+        x = one + k * ulp + ulp / 2
+        got = rn_ref.parse(oracle.rn_op(0, p, _hex(one + k * ulp), _hex(ulp / 2)))
+        assert got == (one + (k + (k % 2)) * ulp), (p, k)
+
+
+@pytest.mark.parametrize("p", [24, 53, 80, 128])
+@pytest.mark.parametrize("fractal", [0, 1, 2, 3])
+def test_mpfr_loop_matches_exact_rounding(oracle, p, fractal):
+    rect, _, _, _, _, _ = cases.scene_rect("test")
+    rng = random.Random(p * 10 + fractal)
+    for _ in range(3):
+        line, ix = rng.randrange(180), rng.randrange(240)
+        xs, ys = oracle.pixel_c(rect, precision=p, width=240, height=180, line=line, ix=ix)
+        x, y = rn_ref.parse(xs), rn_ref.parse(ys)
+        n_or, _, _ = oracle.formula_mpfr(fractal, 60, p, ys, xs, ys, xs,
+                                         _hex(rn_ref.rn(y * y, p)), _hex(rn_ref.rn(x * x, p)))
+        n_py = rn_ref.formula(fractal, 60, p, y, x, y, x, rn_ref.rn(y * y, p), rn_ref.rn(x * x, p))
+        assert n_or == n_py
+
+
+def test_pixel_c_chain_matches_exact_rounding(oracle):
+    for name in ("test", "subdudehue", "satellite"):
+        rect, p, _, _, _, (W, H) = cases.scene_rect(name)
+        xmin, ymax, width = (rn_ref.parse(rect[i]) for i in (0, 2, 3))
+        for line, ix in ((0, 0), (H - 1, W - 1), (H // 2, 17), (3, W // 3)):
+            xs, ys = oracle.pixel_c(rect, precision=p, width=W, height=H, line=line, ix=ix)
+            x, y = rn_ref.pixel_c(xmin, ymax, width, p, W, line, ix)
+            assert rn_ref.parse(xs) == x and rn_ref.parse(ys) == y, (name, line, ix)
+
+
+@pytest.mark.parametrize("args", [
+    ("-0.75", "0", "3.0", 53, 240, 180),
+    ("-0.743643887037158704752191506114774", "0.131825904205311970493132056385139",
+     "1e-25", 128, 640, 480),
+    ("-0.75", "0", "1e-290", 1024, 1920, 1080),
+    ("0.25", "-0.5", "0.001", 176, 100, 300),      # tall image branch
+    ("-1.25", "0.0625", "2.5e-7", 80, 333, 333),   # square
+])
+def test_rect_chain_matches_exact_rounding(oracle, args):
+    got = [rn_ref.parse(s) for s in oracle.rect_from_center(*args)]
+    assert got == list(rn_ref.rect_from_center(*args))
+
+
+def test_ld_loop_is_rn64(oracle):
+    """x87 long double (64-bit significand, round to nearest) == RN at p=64."""
+    rng = random.Random(7)
+    for _ in range(200):
+        cre = Fraction(rng.getrandbits(64), 1 << 64) * 3 - 2
+        cim = Fraction(rng.getrandbits(64), 1 << 64) * 2 - 1
+        cre, cim = rn_ref.rn(cre, 64), rn_ref.rn(cim, 64)
+        re2, im2 = rn_ref.rn(cre * cre, 64), rn_ref.rn(cim * cim, 64)
+        ld = [_hex(v) for v in (cim, cre, cim, cre, im2, re2)]
+        for fractal in range(4):
+            n_ld = oracle.formula_ld(fractal, 300, *ld)
+            n_rn, _, _ = oracle.formula_mpfr(fractal, 300, 64, *[_hex(v) for v in (cim, cre, cim, cre, im2, re2)])
+            assert n_ld == n_rn
+
+
+def _py_double_loop(fractal, depth, wim, wre, c_im, c_re, wim2, wre2):
+    for wz in range(1, depth + 1):
+        if fractal == 1:
+            wim = 2.0 * abs(wre * wim) + c_im
+        else:
+            wim = 2.0 * wre * wim + c_im
+        if fractal == 2 or (fractal == 3 and wz % 2):
+            wre = abs(wre2 - wim2) + c_re
+        else:
+            wre = wre2 - wim2 + c_re
+        wim2 = wim * wim
+        wre2 = wre * wre
+        if wim2 + wre2 > 4.0:
+            return wz
+    return 0
+
+
+def test_double_loop_matches_python_floats(oracle):
+    import mdz_amd.workloads as WL
+    v = WL.config1_view()
+    rect = [s.decode() for s in v._keep[:4]]
+    rows = oracle.render(rect, precision=64, depth=1000, width=240, height=180,
+                         path=oracle.PATH_DOUBLE, lines=(0, 12, 15), cols=(0, 240))
+    xmin, xmax, ymax = (float.fromhex(oracle.pixel_c(rect, precision=64, width=240, height=180,
+                                                     line=0, ix=0, path=2)[0]), None, None)
+    for k, line in enumerate(range(0, 180, 15)):
+        for ix in range(0, 240, 7):
+            xs, ys = oracle.pixel_c(rect, precision=64, width=240, height=180, line=line, ix=ix,
+                                    path=oracle.PATH_DOUBLE)
+            x, y = float.fromhex(xs), float.fromhex(ys)
+            n = _py_double_loop(0, 1000, y, x, y, x, y * y, x * x)
+            assert rows[k, ix] == n
+            for fr in (1, 2, 3):
+                assert oracle.formula_d(fr, 1000, y, x, y, x, y * y, x * x) == \
+                    _py_double_loop(fr, 1000, y, x, y, x, y * y, x * x)
+
+
+@pytest.mark.parametrize("c,count", [
+    ((0.0, 0.0), 0), ((-1.0, 0.0), 0),          # fixed point / 2-cycle
+    ((-2.0, 0.0), 0),                           # |z|^2 == 4 forever: not > 4
+    ((1.0, 0.0), 2),                            # z: 1, 2 (|z|^2 = 4), 5
+    ((2.0, 0.0), 1), ((0.0, 1.0), 0),           # escapes at once / preperiodic
+    ((0.25, 0.0), 0), ((-0.1, 0.1), 0),         # cardioid cusp / interior
+])
+def test_closed_forms_every_path(oracle, c, count):
+    x, y = c
+    assert oracle.formula_d(0, 500, y, x, y, x, y * y, x * x) == count
+    ld = [float(v).hex() for v in (y, x, y, x, y * y, x * x)]
+    assert oracle.formula_ld(0, 500, *ld) == count
+    for p in (2, 53, 128, 1024):
+        args = [_hex(Fraction(v)) for v in (y, x, y, x, y * y, x * x)]
+        assert oracle.formula_mpfr(0, 500, p, *args)[0] == count
+
+
+def test_oracle_matches_golden(oracle, golden):
+    """The restated loops reproduce the maps MDZ's own loops produced."""
+    for meta, expect in golden:
+        vd = meta["view"]
+        got = oracle.render(vd["rect"], precision=vd["precision"], depth=vd["depth"],
+                            width=vd["width"], height=vd["height"], fractal=vd["fractal"],
+                            family=vd["family"], path=vd["path"], julia=tuple(vd["julia"]),
+                            lines=tuple(meta["lines"]), cols=tuple(meta["cols"]), threads=8)
+        assert np.array_equal(got, expect), meta["name"]
+
+
+@pytest.mark.skipif(REF_MISSING, reason="oracle/_ref is built only where the reference tree exists")
+def test_restated_loops_equal_reference_loops(oracle):
+    """Restated ld/MPFR loops == compiled src/frac_*.c on full rows of several views."""
+    import mdz_amd.workloads as WL
+    v = WL.config1_view()
+    r1 = [s.decode() for s in v._keep[:4]]
+    jobs = [(r1, 64, 1000, 240, 180, 1, fr, 0, (None, None)) for fr in range(4)]
+    for name in ("subdudehue", "test", "floral"):
+        rect, p, depth, _, _, (W, H) = cases.scene_rect(name)
+        jobs += [(rect, p, min(depth, 3000), W, H, 0, fr, 0, (None, None)) for fr in range(4)]
+    J = cases.JULIA
+    jobs += [(J["rect"], J["precision"], 500, 96, 72, 0, fr, 1, J["c"]) for fr in range(4)]
+    for rect, p, depth, W, H, path, fr, fam, jc in jobs:
+        kw = dict(precision=p, depth=depth, width=W, height=H, path=path, fractal=fr,
+                  family=fam, julia=jc, lines=(H // 3, 3, 7), threads=8)
+        oracle.use_reference(False)
+        a = oracle.render(rect, **kw)
+        oracle.use_reference(True)
+        b = oracle.render(rect, **kw)
+        oracle.use_reference(False)
+        assert np.array_equal(a, b), (p, path, fr, fam)
