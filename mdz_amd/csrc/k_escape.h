@@ -236,8 +236,9 @@ __device__ __forceinline__ bool rn_step(rn_state<N>& s, int fractal, int32_t wz,
     add_rn<N>(s.wre, t, s.cre, 0u, k);                       /* wre = t1 + c_re  */
     mul_rn<N, true>(s.wim2, s.wim, s.wim, k);                /* wim2 = wim^2     */
     mul_rn<N, true>(s.wre2, s.wre, s.wre, k);                /* wre2 = wre^2     */
-    add_rn<N>(t, s.wim2, s.wre2, 0u, k);                     /* t1 = wim2 + wre2 */
-    return gt4<N>(t);                                        /* t1 > 4           */
+    /* RN(wim2 + wre2) > 4  <=>  wim2 + wre2 > 4 + 2^(2-p) */
+    const double thresh = 4.0 + ldexp(1.0, 2 - (32 * N - k));
+    return escaped<N>(s.wim2, s.wre2, thresh, k);
 }
 
 template <int N, bool FULL>

@@ -24,13 +24,19 @@
  * is uniform across a wave (one tile, one precision): kclr == 0 takes the
  * word-aligned rounding path, kclr > 0 the masked one.
  *
- * The header is plain C++ apart from the wave vote used to skip stages of the
- * variable shifters that no lane needs; built for the host (tests only) the
- * vote is the lane's own predicate, which skips exactly the same no-op work.
+ * gfx950 mapping: a 32x32+64 multiply-accumulate is one v_mad_u64_u32 whose
+ * carry-out feeds one v_addc_co_u32 into the third accumulator word (the
+ * device form of mac96 below; the host form computes the same function in
+ * portable C).  Word carry chains use v_add_co/v_addc_co via __builtin_addc.
+ * Rare cases (carry through a rounding increment, cancellation of a whole
+ * word, a borrow that flips the sign) run under wave-wide votes, so a wave
+ * only pays for them when one of its lanes needs them.  Built for the host
+ * (tests only) the vote is the lane's own predicate.
  */
 #ifndef MDZG_MP_RN_H
 #define MDZG_MP_RN_H
 
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -66,6 +72,21 @@ MP_HD uint32_t funnel_r(uint32_t hi, uint32_t lo, uint32_t s)
     return (uint32_t)((((uint64_t)hi << 32) | lo) >> (s & 31u));
 }
 
+/* a + b + cin, carry to cout (v_add_co_u32 / v_addc_co_u32) */
+MP_HD uint32_t add_cc(uint32_t a, uint32_t b, uint32_t cin, uint32_t& cout)
+{
+#if defined(__clang__)
+    unsigned int co;
+    const uint32_t r = __builtin_addc(a, b, cin, &co);
+    cout = co;
+    return r;
+#else
+    const uint64_t t = (uint64_t)a + b + cin;
+    cout = (uint32_t)(t >> 32);
+    return (uint32_t)t;
+#endif
+}
+
 /* number of power-of-two stages of a shifter covering shifts 0..n */
 constexpr int stages(int n)
 {
@@ -96,50 +117,59 @@ MP_HD bool is_zero(const mp<N>& a)
 /* Rounding of a normalised candidate R (N words) with a guard word G   */
 /* (the 32 bits right below R) and a sticky flag S (anything below G).  */
 /* ------------------------------------------------------------------ */
+
+/* R += inc << bit at word w0 (runtime uniform), carries propagate upward;
+   a carry out of the top word means R was all ones: R = 2^(32N-1), e + 1. */
+template <int N>
+MP_HD void add_ulp(uint32_t* R, int32_t& e, uint32_t inc_word0)
+{
+    uint32_t c;
+    R[0] = add_cc(R[0], inc_word0, 0u, c);
+    if (wave_any(c != 0)) {          /* R[0] wrapped: rare */
+#pragma unroll
+        for (int j = 1; j < N; ++j) R[j] = add_cc(R[j], 0u, c, c);
+        if (c) {
+            R[N - 1] = 0x80000000u;
+            e += 1;
+        }
+    }
+}
+
 template <int N>
 MP_HD void round_full(uint32_t* R, uint32_t G, uint32_t S, int32_t& e)
 {
-    uint32_t below = (G << 1) | S;
-    uint32_t inc = (G >> 31) & (uint32_t)((below != 0) | (R[0] & 1u));
-    uint32_t c = inc;
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        uint32_t t = R[j] + c;
-        c = (uint32_t)(t < c);
-        R[j] = t;
-    }
-    if (c) {            /* R was all ones: now 2^(32N) -> 2^(32N-1), e+1 */
-        R[N - 1] = 0x80000000u;
-        e += 1;
-    }
+    const uint32_t below = (G << 1) | S;
+    const uint32_t inc = (G >> 31) & (uint32_t)((below != 0) | (R[0] & 1u));
+    add_ulp<N>(R, e, inc);
 }
 
 /* kclr = 32N - p in [1, 32N-1]: clear the low kclr bits of R, round at bit kclr */
 template <int N>
 MP_HD void round_masked(uint32_t* R, uint32_t G, uint32_t S, int32_t& e, int kclr)
 {
-    const int wi = kclr >> 5, bi = kclr & 31;          /* lsb of the kept part */
-    const int rr = kclr - 1, rw = rr >> 5, rb = rr & 31; /* round bit          */
+    const int wi = kclr >> 5, bi = kclr & 31;            /* lsb of the kept part */
+    const int rr = kclr - 1, rw = rr >> 5, rb = rr & 31; /* round bit            */
     uint32_t rbit = 0, st = G | S, lsb = 0;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        uint32_t keep = (j > wi) ? 0xffffffffu : (j == wi ? (0xffffffffu << bi) : 0u);
-        uint32_t rm = (j == rw) ? (1u << rb) : 0u;
-        uint32_t sm = (j < rw) ? 0xffffffffu : (j == rw ? ((1u << rb) - 1u) : 0u);
-        uint32_t lm = (j == wi) ? (1u << bi) : 0u;
+        const uint32_t keep = (j > wi) ? 0xffffffffu : (j == wi ? (0xffffffffu << bi) : 0u);
+        const uint32_t rm = (j == rw) ? (1u << rb) : 0u;
+        const uint32_t sm = (j < rw) ? 0xffffffffu : (j == rw ? ((1u << rb) - 1u) : 0u);
+        const uint32_t lm = (j == wi) ? (1u << bi) : 0u;
         rbit |= R[j] & rm;
         st |= R[j] & sm;
         lsb |= R[j] & lm;
         R[j] &= keep;
     }
-    uint32_t inc = (uint32_t)(rbit != 0) & (uint32_t)((st != 0) | (lsb != 0));
+    const uint32_t inc = (uint32_t)(rbit != 0) & (uint32_t)((st != 0) | (lsb != 0));
+    /* add inc << kclr: word wi gets inc << bi, carries ripple upward
+       (words below wi are zero and receive nothing) */
     uint32_t c = 0;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        uint32_t add = (j == wi) ? (inc << bi) : 0u;
-        uint64_t t = (uint64_t)R[j] + add + c;
-        R[j] = (uint32_t)t;
-        c = (uint32_t)(t >> 32);
+        const uint32_t add = (j == wi) ? (inc << bi) : 0u;
+        const uint32_t cin = (j > wi) ? c : 0u;
+        R[j] = add_cc(R[j], add, cin, c);
     }
     if (c) {
         R[N - 1] = 0x80000000u;
@@ -160,76 +190,84 @@ MP_HD void round_p(uint32_t* R, uint32_t G, uint32_t S, int32_t& e, int kclr)
 /* Significand product, column by column (product scanning).            */
 /* Outputs the top N+1 words of the 2N-word product (H[0] = word N-1,   */
 /* the guard word; H[N] = word 2N-1) and the OR of words 0..N-2.        */
-/* Each column sum is kept in a 96-bit accumulator (c2:c1:c0).          */
+/* Each column sum is kept in a 96-bit accumulator (c2 : acc).          */
 /* ------------------------------------------------------------------ */
-MP_HD void mac96(uint32_t a, uint32_t b, uint32_t& c0, uint32_t& c1, uint32_t& c2)
+MP_HD void mac96(uint32_t a, uint32_t b, uint64_t& acc, uint32_t& c2)
 {
-    uint64_t acc = ((uint64_t)c1 << 32) | c0;
-    uint64_t p = (uint64_t)a * b;
-    uint64_t t = acc + p;
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t cc;
+    asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
+        "v_addc_co_u32 %2, %1, 0, %2, %1"
+        : "+v"(acc), "=&s"(cc), "+v"(c2)
+        : "v"(a), "v"(b));
+#else
+    const uint64_t p = (uint64_t)a * b;
+    const uint64_t t = acc + p;
     c2 += (uint32_t)(t < p);
-    c0 = (uint32_t)t;
-    c1 = (uint32_t)(t >> 32);
+    acc = t;
+#endif
 }
 
 template <int N>
 MP_HD void mant_mul(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t& S)
 {
-    uint32_t c0 = 0, c1 = 0, c2 = 0, st = 0;
+    uint64_t acc = 0;
+    uint32_t c2 = 0, st = 0;
 #pragma unroll
     for (int k = 0; k < 2 * N - 1; ++k) {
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             const int j = k - i;
             if (j >= 0 && j < N)
-                mac96(a[i], b[j], c0, c1, c2);
+                mac96(a[i], b[j], acc, c2);
         }
         if (k < N - 1)
-            st |= c0;
+            st |= (uint32_t)acc;
         else
-            H[k - (N - 1)] = c0;
-        c0 = c1;
-        c1 = c2;
+            H[k - (N - 1)] = (uint32_t)acc;
+        acc = (acc >> 32) | ((uint64_t)c2 << 32);
         c2 = 0;
     }
-    H[N] = c0;
+    H[N] = (uint32_t)acc;
     S = (uint32_t)(st != 0);
 }
 
-/* Square: off-diagonal products once, doubled, plus the diagonal. */
+/* Square with the off-diagonal products formed once and doubled per column
+   (n(n+1)/2 multiply-accumulates); pays off from N >= 24 words. */
 template <int N>
-MP_HD void mant_sqr(const uint32_t* a, uint32_t* H, uint32_t& S)
+MP_HD void mant_sqr_sym(const uint32_t* a, uint32_t* H, uint32_t& S)
 {
-    uint32_t c0 = 0, c1 = 0, c2 = 0, st = 0;
+    uint64_t acc = 0;
+    uint32_t c2 = 0, st = 0;
 #pragma unroll
     for (int k = 0; k < 2 * N - 1; ++k) {
-        uint32_t o0 = 0, o1 = 0, o2 = 0;
+        uint64_t o = 0;
+        uint32_t o2 = 0;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             const int j = k - i;
             if (j > i && j < N)
-                mac96(a[i], a[j], o0, o1, o2);
+                mac96(a[i], a[j], o, o2);
         }
-        /* acc += 2*off */
-        uint32_t d2 = (o2 << 1) | (o1 >> 31);
-        uint32_t d1 = (o1 << 1) | (o0 >> 31);
-        uint32_t d0 = o0 << 1;
-        uint64_t t0 = (uint64_t)c0 + d0;
-        uint64_t t1 = (uint64_t)c1 + d1 + (t0 >> 32);
-        c0 = (uint32_t)t0;
-        c1 = (uint32_t)t1;
-        c2 = c2 + d2 + (uint32_t)(t1 >> 32);
+        /* acc += 2 * (o2:o) */
+        const uint32_t d0 = (uint32_t)o << 1;
+        const uint32_t d1 = funnel_r((uint32_t)(o >> 32), (uint32_t)o, 31u);
+        const uint32_t d2 = funnel_r(o2, (uint32_t)(o >> 32), 31u);
+        uint32_t cc;
+        const uint32_t l0 = add_cc((uint32_t)acc, d0, 0u, cc);
+        const uint32_t l1 = add_cc((uint32_t)(acc >> 32), d1, cc, cc);
+        c2 = c2 + d2 + cc;
+        acc = ((uint64_t)l1 << 32) | l0;
         if ((k & 1) == 0)
-            mac96(a[k >> 1], a[k >> 1], c0, c1, c2);
+            mac96(a[k >> 1], a[k >> 1], acc, c2);
         if (k < N - 1)
-            st |= c0;
+            st |= (uint32_t)acc;
         else
-            H[k - (N - 1)] = c0;
-        c0 = c1;
-        c1 = c2;
+            H[k - (N - 1)] = (uint32_t)acc;
+        acc = (acc >> 32) | ((uint64_t)c2 << 32);
         c2 = 0;
     }
-    H[N] = c0;
+    H[N] = (uint32_t)acc;
     S = (uint32_t)(st != 0);
 }
 
@@ -239,17 +277,17 @@ MP_HD void mul_rn(mp<N>& r, const mp<N>& a, const mp<N>& b, int kclr)
 {
     uint32_t H[N + 1];
     uint32_t S;
-    if (SQR)
-        mant_sqr<N>(a.m, H, S);
+    if (SQR && N >= 24)
+        mant_sqr_sym<N>(a.m, H, S);
     else
-        mant_mul<N>(a.m, b.m, H, S);
+        mant_mul<N>(a.m, SQR ? a.m : b.m, H, S);
     int32_t e = SQR ? (a.e + a.e) : (a.e + b.e);
     const uint32_t s = SQR ? 0u : (a.s ^ b.s);
     /* product of normalised values is >= 2^(64N-2): at most one left shift */
     const bool sh = (H[N] >> 31) == 0;
 #pragma unroll
     for (int j = N; j >= 1; --j)
-        H[j] = sh ? ((H[j] << 1) | (H[j - 1] >> 31)) : H[j];
+        H[j] = sh ? funnel_r(H[j], H[j - 1], 31u) : H[j];
     H[0] = sh ? (H[0] << 1) : H[0];
     e -= sh ? 1 : 0;
     round_p<N>(H + 1, H[0], S, e, kclr);
@@ -266,7 +304,7 @@ MP_HD void mul_rn(mp<N>& r, const mp<N>& a, const mp<N>& b, int kclr)
 template <int N>
 MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb, int kclr)
 {
-    /* order by exponent: a gets the larger one */
+    /* order by exponent: A gets the larger one */
     const bool sw = b_in.e > a_in.e;
     uint32_t A[N], B[N];
 #pragma unroll
@@ -277,11 +315,11 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
     const uint32_t sbin = b_in.s ^ flipb;
     const uint32_t sa = sw ? sbin : a_in.s;
     const uint32_t sb = sw ? a_in.s : sbin;
-    const int32_t ea = sw ? b_in.e : a_in.e;
+    int32_t e = sw ? b_in.e : a_in.e;
     const int32_t eb = sw ? a_in.e : b_in.e;
 
     /* align B right by d bits into N+1 words X (X[0] = guard word) + sticky */
-    uint32_t d = (uint32_t)(ea - eb);
+    uint32_t d = (uint32_t)(e - eb);
     if (d > 32u * (N + 1)) d = 32u * (N + 1);
     const uint32_t ws = d >> 5, bs = d & 31u;
     uint32_t X[N + 1];
@@ -290,11 +328,11 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
     for (int j = 1; j < N; ++j) X[j] = funnel_r(B[j], B[j - 1], bs);
     X[N] = B[N - 1] >> bs;
     uint32_t S = 0;
+    if (wave_any(ws != 0)) {
 #pragma unroll
-    for (int st = 0; st < stages(N + 1); ++st) {
-        const int t = 1 << st;
-        const bool c = (ws & (uint32_t)t) != 0;
-        if (wave_any(c)) {
+        for (int st = 0; st < stages(N + 1); ++st) {
+            const int t = 1 << st;
+            const bool c = (ws & (uint32_t)t) != 0;
             uint32_t lost = 0;
 #pragma unroll
             for (int j = 0; j < t && j <= N; ++j) lost |= X[j];
@@ -305,8 +343,8 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
             }
             S |= c ? lost : 0u;
         }
+        S = (uint32_t)(S != 0);
     }
-    S = (uint32_t)(S != 0);
 
     /* T = A*2^32 +/- (X + S): one carry chain, X complemented for a
        subtraction with carry-in 1 - S (A - X - S = A + ~X + 1 - S). */
@@ -314,55 +352,35 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
     const uint32_t msk = 0u - sub;
     uint32_t T[N + 1];
     uint32_t c = sub & (S ^ 1u);
-    {
-        uint64_t t = (uint64_t)(X[0] ^ msk) + c;
-        T[0] = (uint32_t)t;
-        c = (uint32_t)(t >> 32);
-    }
+    T[0] = add_cc(X[0] ^ msk, 0u, c, c);
 #pragma unroll
-    for (int j = 1; j <= N; ++j) {
-        uint64_t t = (uint64_t)A[j - 1] + (X[j] ^ msk) + c;
-        T[j] = (uint32_t)t;
-        c = (uint32_t)(t >> 32);
-    }
-    int32_t e = ea;
+    for (int j = 1; j <= N; ++j) T[j] = add_cc(A[j - 1], X[j] ^ msk, c, c);
     uint32_t s = sa;
 
-    /* add with carry-out: shift right one bit */
+    /* same signs with a carry out: shift right one bit */
     const bool rs = (sub == 0) & (c != 0);
-    S |= rs ? (T[0] & 1u) : 0u;
+    if (wave_any(rs)) {
+        S |= rs ? (T[0] & 1u) : 0u;
 #pragma unroll
-    for (int j = 0; j < N; ++j) T[j] = rs ? funnel_r(T[j + 1], T[j], 1u) : T[j];
-    T[N] = rs ? ((T[N] >> 1) | 0x80000000u) : T[N];
-    e += rs ? 1 : 0;
+        for (int j = 0; j < N; ++j) T[j] = rs ? funnel_r(T[j + 1], T[j], 1u) : T[j];
+        T[N] = rs ? ((T[N] >> 1) | 0x80000000u) : T[N];
+        e += rs ? 1 : 0;
+    }
 
-    /* subtract with borrow (only when d == 0, so S == 0): negate */
+    /* a borrow (only possible when d == 0, so S == 0): negate, flip sign */
     const bool ng = (sub != 0) & (c == 0);
     if (wave_any(ng)) {
         uint32_t cc = ng ? 1u : 0u;
         const uint32_t nm = ng ? 0xffffffffu : 0u;
 #pragma unroll
-        for (int j = 0; j <= N; ++j) {
-            uint64_t t = (uint64_t)(T[j] ^ nm) + cc;
-            T[j] = (uint32_t)t;
-            cc = (uint32_t)(t >> 32);
-        }
+        for (int j = 0; j <= N; ++j) T[j] = add_cc(T[j] ^ nm, 0u, cc, cc);
         s = ng ? (s ^ 1u) : s;
     }
 
-    /* normalise: 0 or 1 bit of left shift is the common case */
-    const uint32_t top = T[N];
-    const bool l1 = (top >> 30) == 1u;
-    const bool big = (top >> 30) == 0u;   /* zero, or cancellation of >= 2 bits */
-#pragma unroll
-    for (int j = N; j >= 1; --j) T[j] = l1 ? funnel_r(T[j], T[j - 1], 31u) : T[j];
-    T[0] = l1 ? (T[0] << 1) : T[0];
-    e -= l1 ? 1 : 0;
-
+    /* cancellation of the whole top word (exact case, S == 0): word shift */
     bool zero = false;
-    if (wave_any(big)) {
-        /* exact (S == 0) result with >= 2 leading zero bits, or zero */
-        uint32_t nz = 0, lzw = 0;
+    if (wave_any(T[N] == 0)) {
+        uint32_t lzw = 0;
         bool found = false;
 #pragma unroll
         for (int j = N; j >= 0; --j) {
@@ -370,29 +388,28 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
             lzw = here ? (uint32_t)(N - j) : lzw;
             found = found || (T[j] != 0);
         }
-        nz = found ? 1u : 0u;
-        zero = big && !nz;
-        const uint32_t wsh = big ? lzw : 0u;
+        zero = !found;
 #pragma unroll
         for (int st = 0; st < stages(N); ++st) {
             const int t = 1 << st;
-            const bool cnd = (wsh & (uint32_t)t) != 0;
-            if (wave_any(cnd)) {
+            const bool cnd = (lzw & (uint32_t)t) != 0;
 #pragma unroll
-                for (int j = N; j >= 0; --j) {
-                    const uint32_t nv = (j - t >= 0) ? T[j - t] : 0u;
-                    T[j] = cnd ? nv : T[j];
-                }
+            for (int j = N; j >= 0; --j) {
+                const uint32_t nv = (j - t >= 0) ? T[j - t] : 0u;
+                T[j] = cnd ? nv : T[j];
             }
         }
-        const uint32_t bsh = big ? clz32(T[N]) & 31u : 0u;
-        if (wave_any(bsh != 0)) {
+        e -= (int32_t)(32u * lzw);
+    }
+
+    /* bit normalisation: left shift by the leading zeros of the top word
+       (0 or 1 unless the subtraction was exact) */
+    const uint32_t lz = clz32(T[N]) & 31u;
+    if (wave_any(lz != 0)) {
 #pragma unroll
-            for (int j = N; j >= 1; --j)
-                T[j] = bsh ? funnel_r(T[j], T[j - 1], 32u - bsh) : T[j];
-            T[0] = bsh ? (T[0] << bsh) : T[0];
-        }
-        e -= big ? (int32_t)(32u * wsh + bsh) : 0;
+        for (int j = N; j >= 1; --j) T[j] = lz ? funnel_r(T[j], T[j - 1], 32u - lz) : T[j];
+        T[0] = T[0] << lz;
+        e -= (int32_t)lz;
     }
 
     round_p<N>(T + 1, T[0], S, e, kclr);
@@ -418,6 +435,34 @@ MP_HD bool gt4(const mp<N>& a)
     for (int j = 0; j < N - 1; ++j) low |= a.m[j];
     const bool above = (a.m[N - 1] != 0x80000000u) | (low != 0);
     return (a.s == 0) & !is_zero(a) & ((a.e > 3) | ((a.e == 3) & above));
+}
+
+/* value of a nonnegative a from its top 64 bits (relative error < 2^-52) */
+template <int N>
+MP_HD double approx_nonneg(const mp<N>& a)
+{
+    const uint32_t hi = a.m[N - 1];
+    const uint32_t lo = (N >= 2) ? a.m[N - 2] : 0u;
+    const double v = (double)hi * 4294967296.0 + (double)lo;
+    return ldexp(v, a.e - 64);
+}
+
+/* RN_p(wim2 + wre2) > 4 for nonnegative wim2, wre2: decided from the top
+   words unless the sum lies within 2^-40 of the threshold 4 + 2^(2-p), in
+   which case the exact sum is rounded (src/frac_mandel.c:47-48). */
+template <int N>
+MP_HD bool escaped(const mp<N>& wim2, const mp<N>& wre2, double thresh, int kclr)
+{
+    const double sa = approx_nonneg<N>(wim2) + approx_nonneg<N>(wre2);
+    const double dlt = sa - thresh;
+    bool esc = dlt > 0.0;
+    const bool unsure = fabs(dlt) <= 0x1p-40;
+    if (wave_any(unsure)) {
+        mp<N> t;
+        add_rn<N>(t, wim2, wre2, 0u, kclr);
+        esc = unsure ? gt4<N>(t) : esc;
+    }
+    return esc;
 }
 
 } /* namespace mdzg */

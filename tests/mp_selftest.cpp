@@ -231,6 +231,59 @@ static void check_chain(int p, long cases)
     mpfr_clears(wre, wim, wre2, wim2, cre, cim, t1, tmp, (mpfr_ptr)0);
 }
 
+// escape decision RN(a+b) > 4 from approx_nonneg + exact fallback, with sums
+// placed on, just above and just below the threshold 4 + 2^(2-p)
+template <int N>
+static void check_escape(int p, long cases)
+{
+    const int kclr = 32 * N - p;
+    const double thresh = 4.0 + ldexp(1.0, 2 - p);
+    mpfr_t A, B, R;
+    mpfr_inits2(32 * N, A, B, (mpfr_ptr)0);
+    mpfr_init2(R, p);
+    for (long it = 0; it < cases; ++it) {
+        mp<N> a, b;
+        rand_op(a, p, -3, 3);
+        a.s = 0;
+        if (is_zero(a)) rand_op(a, p, 1, 2), a.s = 0;
+        int mode = (int)(rnd32() % 3);
+        if (mode == 0) {
+            rand_op(b, p, -3, 3);
+        } else {
+            // b = RN_p(target - a) with target 4 + 2^(2-p) + k * 2^(e) small offsets
+            mpfr_t T;
+            mpfr_init2(T, 4 * 32 * N + 64);
+            to_mpfr(A, a);
+            mpfr_set_ui(T, 4, MPFR_RNDN);
+            mpfr_t q;
+            mpfr_init2(q, 64);
+            mpfr_set_ui_2exp(q, 1, 2 - p, MPFR_RNDN);
+            if (mode == 2) mpfr_add(T, T, q, MPFR_RNDN);
+            long k = (long)(rnd32() % 7) - 3;
+            mpfr_set_si_2exp(q, k, -(long)(rnd32() % (uint32_t)(2 * p)), MPFR_RNDN);
+            mpfr_add(T, T, q, MPFR_RNDN);
+            mpfr_sub(T, T, A, MPFR_RNDN);
+            if (mpfr_sgn(T) <= 0) { mpfr_clears(T, q, (mpfr_ptr)0); continue; }
+            mpfr_t t2;
+            mpfr_init2(t2, p);
+            mpfr_set(t2, T, MPFR_RNDN);
+            from_mpfr(b, t2);
+            mpfr_clears(T, q, t2, (mpfr_ptr)0);
+        }
+        b.s = 0;
+        to_mpfr(A, a);
+        to_mpfr(B, b);
+        mpfr_add(R, A, B, MPFR_RNDN);
+        const bool ref = mpfr_cmp_ui(R, 4) > 0;
+        const bool got = escaped<N>(a, b, thresh, kclr);
+        if (ref != got) {
+            if (fails < 10) fprintf(stderr, "ESCAPE MISMATCH N=%d p=%d mode=%d\n", N, p, mode);
+            ++fails;
+        }
+    }
+    mpfr_clears(A, B, R, (mpfr_ptr)0);
+}
+
 template <int N>
 static void check_n(long cases)
 {
@@ -244,6 +297,7 @@ static void check_n(long cases)
     if (N <= 2)
         for (int p = 1; p <= 32 * N; ++p) check_p<N>(p, cases / 8 + 1);
     for (int i = 0; i < np; ++i) {
+        check_escape<N>(ps[i], cases);
         check_p<N>(ps[i], cases);
         check_chain<N>(ps[i], cases / 20 + 1);
     }
