@@ -198,6 +198,14 @@ __device__ __forceinline__ void load_mp(mp<N>& v, const uint32_t* p)
     v.s = p[N + 1];
 }
 
+/* minimum waves per SIMD the RN kernels are compiled for: small N keep
+   several waves resident, large N get the register file to themselves */
+template <int N>
+constexpr int rn_min_waves()
+{
+    return N >= 24 ? 1 : (N >= 12 ? 2 : 4);
+}
+
 template <int N>
 struct rn_state {
     mp<N> wre, wim, wre2, wim2, cre, cim;
@@ -242,7 +250,7 @@ __device__ __forceinline__ bool rn_step(rn_state<N>& s, int fractal, int32_t wz,
 }
 
 template <int N, bool FULL>
-__global__ __launch_bounds__(256) void k_rn_tile(const mdzg_dtile* tp)
+__global__ __launch_bounds__(256, rn_min_waves<N>()) void k_rn_tile(const mdzg_dtile* tp)
 {
     const mdzg_dtile& t = *tp;
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -265,7 +273,7 @@ __global__ __launch_bounds__(256) void k_rn_tile(const mdzg_dtile* tp)
 }
 
 template <int N, bool FULL>
-__global__ __launch_bounds__(256) void k_rn_persist(const mdzg_dtile* tiles, int ntiles,
+__global__ __launch_bounds__(256, rn_min_waves<N>()) void k_rn_persist(const mdzg_dtile* tiles, int ntiles,
                                                     uint64_t total,
                                                     unsigned long long* counter,
                                                     const volatile int* cancel, int chunk)

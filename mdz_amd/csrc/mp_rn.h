@@ -66,10 +66,40 @@ MP_HD bool wave_any(bool c)
 #endif
 }
 
+/* lane mask of a per-lane predicate (device) / the predicate itself (host) */
+MP_HD uint64_t lane_mask(bool c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __ballot(c);
+#else
+    return c ? 1u : 0u;
+#endif
+}
+
+/* m ? y : x word by word, opaque to the optimiser on the device so that the
+   swap of two operands stays in registers (a plain select of two arrays is
+   turned into a select of pointers and a copy through private memory) */
+MP_HD uint32_t sel_mask(uint64_t m, uint32_t x, uint32_t y)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(y), "s"(m));
+    return r;
+#else
+    return m ? y : x;
+#endif
+}
+
 /* (hi:lo) >> s, low 32 bits, for 0 <= s < 32 (v_alignbit_b32) */
 MP_HD uint32_t funnel_r(uint32_t hi, uint32_t lo, uint32_t s)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    /* the builtin keeps word arrays as words (a 64-bit shift of two adjacent
+       words is fused into a 64-bit load that pins the array in memory) */
+    return __builtin_amdgcn_alignbit(hi, lo, s & 31u);
+#else
     return (uint32_t)((((uint64_t)hi << 32) | lo) >> (s & 31u));
+#endif
 }
 
 /* a + b + cin, carry to cout (v_add_co_u32 / v_addc_co_u32) */
@@ -208,19 +238,53 @@ MP_HD void mac96(uint32_t a, uint32_t b, uint64_t& acc, uint32_t& c2)
 #endif
 }
 
+/* independent accumulators per column: long columns are split into this many
+   dependency chains so one wave keeps several multiply-accumulates in flight */
+template <int N>
+constexpr int mac_chains()
+{
+    return N >= 24 ? 4 : (N >= 12 ? 2 : 1);
+}
+
+/* (acc, c2) += (acc_b, c2_b) over 96 bits */
+MP_HD void acc96_add(uint64_t& acc, uint32_t& c2, uint64_t accb, uint32_t c2b)
+{
+    uint32_t cc;
+    const uint32_t l0 = add_cc((uint32_t)acc, (uint32_t)accb, 0u, cc);
+    const uint32_t l1 = add_cc((uint32_t)(acc >> 32), (uint32_t)(accb >> 32), cc, cc);
+    c2 = c2 + c2b + cc;
+    acc = ((uint64_t)l1 << 32) | l0;
+}
+
 template <int N>
 MP_HD void mant_mul(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t& S)
 {
+    constexpr int K = mac_chains<N>();
     uint64_t acc = 0;
     uint32_t c2 = 0, st = 0;
 #pragma unroll
     for (int k = 0; k < 2 * N - 1; ++k) {
+        uint64_t xa[K];
+        uint32_t xc[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            xa[q] = q == 0 ? acc : 0;
+            xc[q] = q == 0 ? c2 : 0;
+        }
+        int n = 0;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             const int j = k - i;
-            if (j >= 0 && j < N)
-                mac96(a[i], b[j], acc, c2);
+            if (j >= 0 && j < N) {
+                mac96(a[i], b[j], xa[n % K], xc[n % K]);
+                ++n;
+            }
         }
+#pragma unroll
+        for (int q = 1; q < K; ++q)
+            if (q < n) acc96_add(xa[0], xc[0], xa[q], xc[q]);
+        acc = xa[0];
+        c2 = xc[0];
         if (k < N - 1)
             st |= (uint32_t)acc;
         else
@@ -237,18 +301,32 @@ MP_HD void mant_mul(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t&
 template <int N>
 MP_HD void mant_sqr_sym(const uint32_t* a, uint32_t* H, uint32_t& S)
 {
+    constexpr int K = mac_chains<N>();
     uint64_t acc = 0;
     uint32_t c2 = 0, st = 0;
 #pragma unroll
     for (int k = 0; k < 2 * N - 1; ++k) {
-        uint64_t o = 0;
-        uint32_t o2 = 0;
+        uint64_t xa[K];
+        uint32_t xc[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            xa[q] = 0;
+            xc[q] = 0;
+        }
+        int n = 0;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             const int j = k - i;
-            if (j > i && j < N)
-                mac96(a[i], a[j], o, o2);
+            if (j > i && j < N) {
+                mac96(a[i], a[j], xa[n % K], xc[n % K]);
+                ++n;
+            }
         }
+#pragma unroll
+        for (int q = 1; q < K; ++q)
+            if (q < n) acc96_add(xa[0], xc[0], xa[q], xc[q]);
+        const uint64_t o = xa[0];
+        const uint32_t o2 = xc[0];
         /* acc += 2 * (o2:o) */
         const uint32_t d0 = (uint32_t)o << 1;
         const uint32_t d1 = funnel_r((uint32_t)(o >> 32), (uint32_t)o, 31u);
@@ -306,11 +384,12 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
 {
     /* order by exponent: A gets the larger one */
     const bool sw = b_in.e > a_in.e;
+    const uint64_t swm = lane_mask(sw);
     uint32_t A[N], B[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        A[j] = sw ? b_in.m[j] : a_in.m[j];
-        B[j] = sw ? a_in.m[j] : b_in.m[j];
+        A[j] = sel_mask(swm, a_in.m[j], b_in.m[j]);
+        B[j] = sel_mask(swm, b_in.m[j], a_in.m[j]);
     }
     const uint32_t sbin = b_in.s ^ flipb;
     const uint32_t sa = sw ? sbin : a_in.s;
