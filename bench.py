@@ -142,8 +142,8 @@ def main():
         # 1024-bit figure: a band of config 5 rows
         v5 = WL.config5_view()
         _, st5 = ctx.render(v5, lines=(1048, 2), cols=(0, 3840))
-        _, st5 = ctx.render(v5, lines=(1048, 16), cols=(0, 3840))
-        res["rn1024"] = dict(workload="config5 rows 1048-1063 x 3840 samples, p=1024, depth 20000",
+        _, st5 = ctx.render(v5, lines=(1048, 64), cols=(0, 3840))
+        res["rn1024"] = dict(workload="config5 rows 1048-1111 x 3840 samples, p=1024, depth 20000",
                              pixel_iters=st5["pixel_iters"], kernel_ms=st5["kernel_ms"],
                              kernel_gpix_per_s=st5["pixel_iters"] / (st5["kernel_ms"] * 1e-3) / 1e9,
                              frac_peak=2 * 2080 * st5["pixel_iters"] / (st5["kernel_ms"] * 1e-3)

@@ -200,15 +200,70 @@ __device__ __forceinline__ void load_mp(mp<N>& v, const uint32_t* p)
 
 /* minimum waves per SIMD the RN kernels are compiled for: small N keep
    several waves resident, large N get the register file to themselves */
-template <int N>
+template <int N, bool FULL>
 constexpr int rn_min_waves()
 {
-    return N >= 24 ? 1 : (N >= 12 ? 2 : 4);
+    return (N >= 32 && !FULL) ? 1 : (N >= 12 ? 2 : 4);
 }
+
+/* c per lane: registers for small N; for N >= 24 the block's LDS (word w of
+   lane t at [w][t], conflict-free), so the register file holds z and z^2 only
+   and two waves fit per SIMD */
+template <int N>
+constexpr bool c_in_lds()
+{
+    return N >= 24;
+}
+
+template <int N, bool LDS>
+struct c_src;
+
+template <int N>
+struct c_src<N, false> {
+    mp<N> re, im;
+    __device__ __forceinline__ void init(uint32_t*) {}
+    __device__ __forceinline__ void put(const mp<N>& r, const mp<N>& i)
+    {
+        re = r;
+        im = i;
+    }
+    __device__ __forceinline__ const mp<N>& get_re() const { return re; }
+    __device__ __forceinline__ const mp<N>& get_im() const { return im; }
+};
+
+template <int N>
+struct c_src<N, true> {
+    uint32_t* base;   /* this lane's column of the block's LDS area */
+    __device__ __forceinline__ void init(uint32_t* lds) { base = lds + threadIdx.x; }
+    __device__ __forceinline__ void put(const mp<N>& r, const mp<N>& i)
+    {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            base[j * 256] = r.m[j];
+            base[(N + 2 + j) * 256] = i.m[j];
+        }
+        base[N * 256] = (uint32_t)r.e;
+        base[(N + 1) * 256] = r.s;
+        base[(2 * N + 2) * 256] = (uint32_t)i.e;
+        base[(2 * N + 3) * 256] = i.s;
+    }
+    __device__ __forceinline__ mp<N> get(int off) const
+    {
+        mp<N> v;
+#pragma unroll
+        for (int j = 0; j < N; ++j) v.m[j] = base[(off + j) * 256];
+        v.e = (int32_t)base[(off + N) * 256];
+        v.s = base[(off + N + 1) * 256];
+        return v;
+    }
+    __device__ __forceinline__ mp<N> get_re() const { return get(0); }
+    __device__ __forceinline__ mp<N> get_im() const { return get(N + 2); }
+};
 
 template <int N>
 struct rn_state {
-    mp<N> wre, wim, wre2, wim2, cre, cim;
+    mp<N> wre, wim, wre2, wim2;
+    c_src<N, c_in_lds<N>()> c;
 };
 
 template <int N, bool FULL>
@@ -220,11 +275,12 @@ __device__ __forceinline__ void rn_init(rn_state<N>& s, const mdzg_dtile& t, int
     mul_rn<N, true>(s.wre2, s.wre, s.wre, k);      /* src/fractal.c:188 */
     mul_rn<N, true>(s.wim2, s.wim, s.wim, k);      /* src/fractal.c:171 */
     if (t.family == 1) {
-        load_mp<N>(s.cre, t.jc);
-        load_mp<N>(s.cim, t.jc + MDZG_VS(N));
+        mp<N> jr, ji;
+        load_mp<N>(jr, t.jc);
+        load_mp<N>(ji, t.jc + MDZG_VS(N));
+        s.c.put(jr, ji);
     } else {
-        s.cre = s.wre;
-        s.cim = s.wim;
+        s.c.put(s.wre, s.wim);
     }
 }
 
@@ -237,11 +293,11 @@ __device__ __forceinline__ bool rn_step(rn_state<N>& s, int fractal, int32_t wz,
     mul_rn<N, false>(t, s.wre, s.wim, k);                    /* t1 = wre*wim     */
     t.s = (fractal == F_BURNING_SHIP) ? 0u : t.s;            /* BS: |t1|         */
     twice<N>(t);                                             /* t1 *= 2          */
-    add_rn<N>(s.wim, t, s.cim, 0u, k);                       /* wim = t1 + c_im  */
+    add_rn<N>(s.wim, t, s.c.get_im(), 0u, k);                /* wim = t1 + c_im  */
     add_rn<N>(t, s.wre2, s.wim2, 1u, k);                     /* t1 = wre2 - wim2 */
     const bool ab = fractal == F_GENERALIZED_CELTIC || (fractal == F_VARIANT && (wz & 1));
     t.s = ab ? 0u : t.s;                                     /* Celtic: |t1|     */
-    add_rn<N>(s.wre, t, s.cre, 0u, k);                       /* wre = t1 + c_re  */
+    add_rn<N>(s.wre, t, s.c.get_re(), 0u, k);                /* wre = t1 + c_re  */
     mul_rn<N, true>(s.wim2, s.wim, s.wim, k);                /* wim2 = wim^2     */
     mul_rn<N, true>(s.wre2, s.wre, s.wre, k);                /* wre2 = wre^2     */
     /* RN(wim2 + wre2) > 4  <=>  wim2 + wre2 > 4 + 2^(2-p) */
@@ -250,7 +306,7 @@ __device__ __forceinline__ bool rn_step(rn_state<N>& s, int fractal, int32_t wz,
 }
 
 template <int N, bool FULL>
-__global__ __launch_bounds__(256, rn_min_waves<N>()) void k_rn_tile(const mdzg_dtile* tp)
+__global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_tile(const mdzg_dtile* tp)
 {
     const mdzg_dtile& t = *tp;
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -258,7 +314,9 @@ __global__ __launch_bounds__(256, rn_min_waves<N>()) void k_rn_tile(const mdzg_d
     if (idx >= npix)
         return;
     const int row = (int)(idx / (uint32_t)t.ncols), col = (int)(idx % (uint32_t)t.ncols);
+    __shared__ uint32_t c_lds[c_in_lds<N>() ? 2 * (N + 2) * 256 : 1];
     rn_state<N> s;
+    s.c.init(c_lds);
     rn_init<N, FULL>(s, t, row, col);
     const int fractal = t.fractal, kclr = t.kclr;
     const int32_t depth = t.depth;
@@ -273,7 +331,7 @@ __global__ __launch_bounds__(256, rn_min_waves<N>()) void k_rn_tile(const mdzg_d
 }
 
 template <int N, bool FULL>
-__global__ __launch_bounds__(256, rn_min_waves<N>()) void k_rn_persist(const mdzg_dtile* tiles, int ntiles,
+__global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_persist(const mdzg_dtile* tiles, int ntiles,
                                                     uint64_t total,
                                                     unsigned long long* counter,
                                                     const volatile int* cancel, int chunk)
@@ -283,7 +341,9 @@ __global__ __launch_bounds__(256, rn_min_waves<N>()) void k_rn_persist(const mdz
     uint32_t round = 0;
     int32_t wz = 0, depth = 0;
     uint32_t* out = nullptr;
+    __shared__ uint32_t c_lds[c_in_lds<N>() ? 2 * (N + 2) * 256 : 1];
     rn_state<N> s;
+    s.c.init(c_lds);
     for (;;) {
         bool any;
         const uint64_t g = claim(!active, counter, any);
