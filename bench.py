@@ -71,6 +71,36 @@ def cpu_baseline(rect, precision, depth, W, H, line_step, threads):
                        f"pixel-iterations) in {el:.2f} s, {threads} threads claiming rows")
 
 
+def config3_batched(ctx, M, WL, np, ntiles=256):
+    """Config 3: the seeded tiles in one persistent launch per arithmetic class
+    (concurrent classes) against one plain launch per tile; counts must match."""
+    specs = WL.config3_specs(ntiles)
+    t0 = time.perf_counter()
+    tiles = [(WL.spec_view(sp), None, None) for sp in specs]
+    log(f"config 3: {len(tiles)} tile views in {time.perf_counter() - t0:.2f} s")
+    outs_p, st_p = ctx.render_batch(tiles, M.BATCH_PERSISTENT)
+    log(f"config 3 persistent: {st_p['kernel_ms']:.0f} ms kernel, {st_p['wall_ms']:.0f} ms wall")
+    outs_t, st_t = ctx.render_batch(tiles, M.BATCH_PER_TILE)
+    log(f"config 3 per-tile:   {st_t['kernel_ms']:.0f} ms kernel, {st_t['wall_ms']:.0f} ms wall")
+    same = all(np.array_equal(x, y) for x, y in zip(outs_p, outs_t))
+    by = {}
+    for sp in specs:
+        by[sp["precision"]] = by.get(sp["precision"], 0) + 1
+    gp = st_p["pixel_iters"] / (st_p["wall_ms"] * 1e-3) / 1e9
+    gt = st_t["pixel_iters"] / (st_t["wall_ms"] * 1e-3) / 1e9
+    return dict(
+        metric=METRIC, value=gp, steps=1, warmup=0, ms_per_step=st_p["wall_ms"],
+        dtype="u32 limbs RN p in {64..1024} + f64",
+        config=dict(workload=f"config3 {len(specs)} seeded tiles (seed 0x4D445A), sides "
+                    "64-512, log10 width U[-250,-2], depth log-U[1e3,5e4], 60/20/20 "
+                    "Mandelbrot/Burning Ship/Celtic", precision_classes=by,
+                    pixel_iters=st_p["pixel_iters"]),
+        batched=dict(persistent_gpix_per_s=gp, per_tile_gpix_per_s=gt, speedup=gp / gt,
+                     persistent_kernel_ms=st_p["kernel_ms"], per_tile_kernel_ms=st_t["kernel_ms"],
+                     launches_persistent=st_p["launches"], launches_per_tile=st_t["launches"],
+                     identical_counts=bool(same)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +185,8 @@ def main():
             rect = [s.decode() for s in v._keep[:4]]
             res["cpu_baseline"] = cpu_baseline(rect, 128, 5000, 640, 480, 4, host_cores())
             log(f"cpu baseline: {res['cpu_baseline']['value']:.4f} G pixel-iter/s")
+    elif a.config == "3":
+        res.update(config3_batched(ctx, M, WL, np))
     else:
         raise SystemExit(f"unknown --config {a.config}")
 

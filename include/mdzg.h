@@ -124,6 +124,17 @@ enum { MDZG_BATCH_PERSISTENT = 0,   /* one persistent launch per arithmetic clas
 int mdzg_render_batch(mdzg_ctx* ctx, const mdzg_tile* tiles, int ntiles, int mode,
                       mdzg_stats* stats);
 
+/* ---- tuning (no counterpart in the reference) ---------------------------
+   Kernel ids: mdzg_kernel_id(path, precision) gives the kernel a view uses
+   (0 = FP64; the RN kernels by word count and exact/ragged precision).
+   blocks_per_cu: resident 256-thread blocks per CU the persistent grid is
+   sized for (<= the kernel's occupancy limit, reported as max_blocks_per_cu);
+   chunk: iterations a lane runs between queue refills.  0 keeps a value. */
+int mdzg_kernel_id(int path, long precision);
+int mdzg_tune_set(mdzg_ctx* ctx, int kernel_id, int blocks_per_cu, int chunk);
+int mdzg_tune_get(mdzg_ctx* ctx, int kernel_id, int* blocks_per_cu, int* chunk,
+                  int* max_blocks_per_cu);
+
 /* ---- asynchronous form (submit / poll / wait / cancel) ---------------- */
 typedef struct mdzg_job mdzg_job;
 int  mdzg_submit(mdzg_ctx* ctx, const mdzg_tile* tiles, int ntiles, mdzg_job** out);

@@ -92,6 +92,9 @@ def lib():
         L.mdzg_frame_stop.argtypes = [vp]
         L.mdzg_frame_stop.restype = None
         L.mdzg_frame_close.argtypes = [vp, P(Stats)]
+        L.mdzg_kernel_id.argtypes = [ctypes.c_int, ctypes.c_long]
+        L.mdzg_tune_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.mdzg_tune_get.argtypes = [vp, ctypes.c_int] + [P(ctypes.c_int)] * 3
         _lib = L
     return _lib
 
@@ -116,6 +119,11 @@ def rect_from_center(cx, cy, size, precision, user_w, user_h):
     _check(lib().mdzg_rect_from_center(_b(cx), _b(cy), _b(size), precision, user_w, user_h,
                                        *bufs, cap), "rect_from_center")
     return tuple(b.value.decode() for b in bufs)
+
+
+def kernel_id(path, precision):
+    """Id of the kernel a view on (path, precision) runs on."""
+    return _check(lib().mdzg_kernel_id(path, precision), "kernel_id")
 
 
 def make_view(rect, *, precision, depth, width, height, fractal=MANDELBROT,
@@ -174,6 +182,16 @@ class Context:
         st = Stats()
         _check(lib().mdzg_render_batch(self._c, arr, n, mode, ctypes.byref(st)), "render_batch")
         return outs, st.as_dict()
+
+    def tune_get(self, kernel_id):
+        """(blocks_per_cu, chunk, max_blocks_per_cu) of a kernel."""
+        vals = [ctypes.c_int() for _ in range(3)]
+        _check(lib().mdzg_tune_get(self._c, kernel_id, *[ctypes.byref(v) for v in vals]),
+               "tune_get")
+        return tuple(v.value for v in vals)
+
+    def tune_set(self, kernel_id, blocks_per_cu=0, chunk=0):
+        _check(lib().mdzg_tune_set(self._c, kernel_id, blocks_per_cu, chunk), "tune_set")
 
     def submit(self, specs):
         """Asynchronous batch: returns a Job."""

@@ -62,7 +62,8 @@ struct mdzg_ctx {
     hipStream_t stream;         /* synchronous batches */
     hipStream_t ctl;            /* cancel-flag writes */
     int* d_cancel_zero;         /* a device int that stays 0 */
-    int occ[1 + 2 * MDZG_NCLASS_N];
+    int occ[1 + 2 * MDZG_NCLASS_N];     /* resident blocks per CU (persistent grid) */
+    int chunk[1 + 2 * MDZG_NCLASS_N];   /* iterations between lane refills */
     pthread_mutex_t lock;       /* serialises users of the shared arena */
     void* d_arena;
     size_t d_cap;

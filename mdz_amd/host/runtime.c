@@ -60,6 +60,8 @@ struct mdzg_job {
     int nkid;
     int kids[1 + 2 * MDZG_NCLASS_N];
     hipEvent_t ev0, ev1;
+    hipEvent_t ev_fork;
+    hipEvent_t ev_cls[1 + 2 * MDZG_NCLASS_N];
     double t_start;
     int launched;
     int done;
@@ -126,6 +128,7 @@ int mdzg_create(int device, mdzg_ctx** out)
         c->occ[k] = mdzg_dev_occupancy(k, 1);
         if (c->occ[k] < 1)
             c->occ[k] = 1;
+        c->chunk[k] = (k == 0) ? 32 : 1;
     }
     pthread_mutex_init(&c->lock, NULL);
     pthread_mutex_init(&f->rt.cache_lock, NULL);
@@ -217,6 +220,48 @@ static hipStream_t pick_stream(mdzg_ctx* c)
     return s;
 }
 
+/* the k-th stream of the pool after `main` (distinct from it for k < NSTREAMS) */
+static hipStream_t side_stream(mdzg_ctx* c, hipStream_t main, int k)
+{
+    ctx_full* f = (ctx_full*)c;
+    int base = 0;
+    for (int i = 0; i < NSTREAMS; ++i)
+        if (f->rt.streams[i] == main)
+            base = i;
+    return f->rt.streams[(base + k) % NSTREAMS];
+}
+
+/* ---- tuning --------------------------------------------------------------- */
+int mdzg_tune_set(mdzg_ctx* c, int kernel_id, int blocks_per_cu, int chunk)
+{
+    if (!c || kernel_id < 0 || kernel_id >= mdzg_dev_nkernels() || blocks_per_cu < 0 ||
+        chunk < 0 || chunk > 4096)
+        return MDZG_ERR_ARG;
+    if (blocks_per_cu > 0) {
+        int cap = mdzg_dev_occupancy(kernel_id, 1);
+        c->occ[kernel_id] = blocks_per_cu < cap ? blocks_per_cu : (cap > 0 ? cap : 1);
+    }
+    if (chunk > 0)
+        c->chunk[kernel_id] = chunk;
+    return MDZG_OK;
+}
+
+int mdzg_tune_get(mdzg_ctx* c, int kernel_id, int* blocks_per_cu, int* chunk, int* max_blocks_per_cu)
+{
+    if (!c || kernel_id < 0 || kernel_id >= mdzg_dev_nkernels())
+        return MDZG_ERR_ARG;
+    if (blocks_per_cu) *blocks_per_cu = c->occ[kernel_id];
+    if (chunk) *chunk = c->chunk[kernel_id];
+    if (max_blocks_per_cu) *max_blocks_per_cu = mdzg_dev_occupancy(kernel_id, 1);
+    return MDZG_OK;
+}
+
+int mdzg_kernel_id(int path, long precision)
+{
+    int nw, kclr;
+    return mdzg_class_of(path, precision, &nw, &kclr);
+}
+
 /* ---- job construction ---------------------------------------------------- */
 static int tile_ok(const mdzg_tile* t)
 {
@@ -229,6 +274,9 @@ static void job_free_mem(mdzg_job* j)
 {
     if (j->ev0) hipEventDestroy(j->ev0);
     if (j->ev1) hipEventDestroy(j->ev1);
+    if (j->ev_fork) hipEventDestroy(j->ev_fork);
+    for (int k = 0; k < 1 + 2 * MDZG_NCLASS_N; ++k)
+        if (j->ev_cls[k]) hipEventDestroy(j->ev_cls[k]);
     if (j->slot >= 0) {
         release(j->ctx, j->slot);
     } else {
@@ -401,6 +449,17 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
             goto done;
         }
         hipEventRecord(j->ev0, j->stream);
+        /* persistent mode: each arithmetic class drains its own queue; the
+           class launches fork onto separate streams so their tails overlap,
+           and join back before the counts are copied out */
+        const int fork = (mode == MDZG_BATCH_PERSISTENT && j->nkid > 1);
+        if (fork) {
+            if (hipEventCreateWithFlags(&j->ev_fork, hipEventDisableTiming) != hipSuccess) {
+                rc = MDZG_ERR_DEVICE;
+                goto done;
+            }
+            hipEventRecord(j->ev_fork, j->stream);
+        }
         int first = 0;
         for (int k = 0; k < j->nkid; ++k) {
             const int kid = j->kids[k];
@@ -412,14 +471,26 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
                 ++cnt;
             }
             if (mode == MDZG_BATCH_PERSISTENT) {
+                hipStream_t st = j->stream;
+                if (fork && k > 0) {
+                    st = side_stream(c, j->stream, k);
+                    hipStreamWaitEvent(st, j->ev_fork, 0);
+                }
                 uint64_t cap = (uint64_t)c->ncu * (uint64_t)c->occ[kid];
                 uint64_t need = (total + 255) / 256;
                 int grid = (int)(need < cap ? need : cap);
-                int chunk = (kid == 0) ? 32 : 1;
+                int chunk = c->chunk[kid];
                 if (mdzg_dev_launch_persist(kid, d_desc + first, cnt, total, d_ctr + k, d_cancel,
-                                            grid, chunk, j->stream) != 0) {
+                                            grid, chunk, st) != 0) {
                     rc = MDZG_ERR_DEVICE;
                     goto done;
+                }
+                if (fork && k > 0) {
+                    if (hipEventCreateWithFlags(&j->ev_cls[k], hipEventDisableTiming) != hipSuccess) {
+                        rc = MDZG_ERR_DEVICE;
+                        goto done;
+                    }
+                    hipEventRecord(j->ev_cls[k], st);
                 }
                 j->launches++;
             } else {
@@ -435,6 +506,9 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
             }
             first += cnt;
         }
+        if (fork)
+            for (int k = 1; k < j->nkid; ++k)
+                hipStreamWaitEvent(j->stream, j->ev_cls[k], 0);
         hipEventRecord(j->ev1, j->stream);
         if (hipMemcpyAsync((char*)j->h + j->off_out, D + j->off_out, j->out_bytes,
                            hipMemcpyDeviceToHost, j->stream) != hipSuccess) {
