@@ -144,6 +144,49 @@ int  mdzg_cancel(mdzg_job* job);    /* device-side stop; untouched counts keep
                                        their previous contents                  */
 void mdzg_job_free(mdzg_job* job);
 
+/* ---- serving: continuous batching of tile requests ----------------------
+   No counterpart in the reference (TODO:15-16 asks for a rendering daemon).
+   A scheduler thread admits requests oldest-first while the HBM budget and
+   each arithmetic class's resident-pixel budget allow, then runs the device
+   in slices: every unfinished pixel of every admitted request resumes from
+   its orbit state kept in HBM and runs up to slice_iters more iterations.
+   Requests arriving meanwhile join at the next slice.  A request's counts
+   are identical alone or in any mix.  The tile's view strings and counts
+   buffer must stay valid until the request is done. */
+enum { MDZG_SERVE_BATCHED = 0,       /* continuous batching (slices)          */
+       MDZG_SERVE_PER_REQUEST = 1 }; /* baseline: one launch per request      */
+
+typedef struct mdzg_server_cfg {
+    int mode;                 /* MDZG_SERVE_*                                    */
+    size_t hbm_budget;        /* device bytes for resident requests (0: 8 GiB)   */
+    double occupancy_budget;  /* resident pixels per class, in units of the lanes
+                                 that class's slice kernel keeps resident (0: 4) */
+    int slice_iters;          /* iterations per pixel per slice (0: by class)    */
+    int max_requests;         /* requests resident at once (0: 4096)             */
+} mdzg_server_cfg;
+
+typedef struct mdzg_req_info {
+    int status;               /* 0 pending, 1 running, 2 done, < 0 error         */
+    double submit_ms, admit_ms, done_ms;   /* host monotonic clock               */
+    double latency_ms;        /* done - submit                                   */
+    double pixel_iters;
+} mdzg_req_info;
+
+typedef struct mdzg_serve_stats {
+    uint64_t submitted, admitted, completed, slices, active, pending;
+    double pixel_iters;
+    size_t hbm_in_use, hbm_peak;
+} mdzg_serve_stats;
+
+typedef struct mdzg_server mdzg_server;
+int  mdzg_server_create(mdzg_ctx* ctx, const mdzg_server_cfg* cfg, mdzg_server** out);
+int  mdzg_server_submit(mdzg_server* srv, const mdzg_tile* tile, uint64_t* id);
+int  mdzg_server_poll(mdzg_server* srv, uint64_t id);     /* 1 done, 0 not yet */
+int  mdzg_server_wait(mdzg_server* srv, uint64_t id, mdzg_req_info* info);
+int  mdzg_server_info(mdzg_server* srv, uint64_t id, mdzg_req_info* info);
+int  mdzg_server_stats(mdzg_server* srv, mdzg_serve_stats* stats);
+void mdzg_server_destroy(mdzg_server* srv);   /* finishes admitted and pending work */
+
 /* ---- line-callback adapter ---------------------------------------------
    A frame holds one view and the caller's raw buffer (image_info.raw_data,
    real_width*real_height ints, src/image_info.c:136-137).  The frame is

@@ -21,6 +21,7 @@ MANDELBROT, BURNING_SHIP, GENERALIZED_CELTIC, VARIANT = 0, 1, 2, 3
 FAMILY_MANDEL, FAMILY_JULIA = 0, 1
 PATH_MPFR, PATH_LD, PATH_DOUBLE = 0, 1, 2
 BATCH_PERSISTENT, BATCH_PER_TILE = 0, 1
+SERVE_BATCHED, SERVE_PER_REQUEST = 0, 1
 MAX_PRECISION = 1024
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "HIP device error",
@@ -59,6 +60,32 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerCfg(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
+                ("occupancy_budget", ctypes.c_double), ("slice_iters", ctypes.c_int),
+                ("max_requests", ctypes.c_int)]
+
+
+class ReqInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("submit_ms", ctypes.c_double),
+                ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("latency_ms", ctypes.c_double), ("pixel_iters", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("admitted", ctypes.c_uint64),
+                ("completed", ctypes.c_uint64), ("slices", ctypes.c_uint64),
+                ("active", ctypes.c_uint64), ("pending", ctypes.c_uint64),
+                ("pixel_iters", ctypes.c_double), ("hbm_in_use", ctypes.c_size_t),
+                ("hbm_peak", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -92,6 +119,14 @@ def lib():
         L.mdzg_frame_stop.argtypes = [vp]
         L.mdzg_frame_stop.restype = None
         L.mdzg_frame_close.argtypes = [vp, P(Stats)]
+        L.mdzg_server_create.argtypes = [vp, P(ServerCfg), P(vp)]
+        L.mdzg_server_submit.argtypes = [vp, P(Tile), P(ctypes.c_uint64)]
+        L.mdzg_server_poll.argtypes = [vp, ctypes.c_uint64]
+        L.mdzg_server_wait.argtypes = [vp, ctypes.c_uint64, P(ReqInfo)]
+        L.mdzg_server_info.argtypes = [vp, ctypes.c_uint64, P(ReqInfo)]
+        L.mdzg_server_stats.argtypes = [vp, P(ServeStats)]
+        L.mdzg_server_destroy.argtypes = [vp]
+        L.mdzg_server_destroy.restype = None
         L.mdzg_kernel_id.argtypes = [ctypes.c_int, ctypes.c_long]
         L.mdzg_tune_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.mdzg_tune_get.argtypes = [vp, ctypes.c_int] + [P(ctypes.c_int)] * 3
@@ -252,3 +287,57 @@ class Frame:
         rc = lib().mdzg_frame_close(self._f, ctypes.byref(st))
         self._f = ctypes.c_void_p()
         return rc, st.as_dict()
+
+
+class Server:
+    """Continuous-batching tile server (mdzg_server_*).  submit() returns an id;
+    wait(id) returns (counts, info).  The server keeps each request's tile, view
+    and counts buffer alive until it is waited for."""
+
+    def __init__(self, ctx, mode=SERVE_BATCHED, hbm_budget=0, occupancy_budget=0.0,
+                 slice_iters=0, max_requests=0):
+        self.ctx = ctx
+        cfg = ServerCfg(mode, hbm_budget, occupancy_budget, slice_iters, max_requests)
+        self._s = ctypes.c_void_p()
+        _check(lib().mdzg_server_create(ctx._c, ctypes.byref(cfg), ctypes.byref(self._s)),
+               "server_create")
+        self._live = {}
+
+    def submit(self, view, lines=None, cols=None):
+        t, out = Context._tile(view, lines, cols)
+        rid = ctypes.c_uint64()
+        _check(lib().mdzg_server_submit(self._s, ctypes.byref(t), ctypes.byref(rid)), "submit")
+        self._live[rid.value] = (t, out, view)
+        return rid.value
+
+    def poll(self, rid):
+        return _check(lib().mdzg_server_poll(self._s, rid), "poll") == 1
+
+    def wait(self, rid):
+        info = ReqInfo()
+        rc = lib().mdzg_server_wait(self._s, rid, ctypes.byref(info))
+        _check(rc, "wait")
+        t, out, view = self._live.pop(rid)
+        return out, info.as_dict()
+
+    def info(self, rid):
+        info = ReqInfo()
+        _check(lib().mdzg_server_info(self._s, rid, ctypes.byref(info)), "info")
+        return info.as_dict()
+
+    def stats(self):
+        st = ServeStats()
+        _check(lib().mdzg_server_stats(self._s, ctypes.byref(st)), "stats")
+        return st.as_dict()
+
+    def close(self):
+        if self._s:
+            lib().mdzg_server_destroy(self._s)
+            self._s = ctypes.c_void_p()
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -148,21 +148,28 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
     uint32_t* out = nullptr;
     fp64_state s;
     s.wre = s.wim = s.wre2 = s.wim2 = s.cre = s.cim = 0.0;
+    bool drained = false;
     for (;;) {
-        bool any;
-        const uint64_t g = claim(!active, counter, any);
-        if (!active && g < total) {
-            const int ti = find_tile(tiles, ntiles, g);
-            const mdzg_dtile& t = tiles[ti];
-            const uint32_t local = (uint32_t)(g - t.pix_begin);
-            const int row = (int)(local / (uint32_t)t.ncols);
-            const int col = (int)(local % (uint32_t)t.ncols);
-            fp64_init(s, t, row, col);
-            fractal = t.fractal;
-            depth = t.depth;
-            out = t.out + local;
-            wz = 0;
-            active = true;
+        if (!drained) {
+            const bool need = !active;
+            bool any;
+            const uint64_t g = claim(need, counter, any);
+            if (need && g < total) {
+                const int ti = find_tile(tiles, ntiles, g);
+                const mdzg_dtile& t = tiles[ti];
+                const uint32_t local = (uint32_t)(g - t.pix_begin);
+                const int row = (int)(local / (uint32_t)t.ncols);
+                const int col = (int)(local % (uint32_t)t.ncols);
+                fp64_init(s, t, row, col);
+                fractal = t.fractal;
+                depth = t.depth;
+                out = t.out + local;
+                wz = 0;
+                active = true;
+            }
+            /* the counter only grows: once one lane reads past the end the wave
+               stops claiming, so no atomics while its last lanes finish */
+            drained = __any(need && g >= total);
         }
         if (!__any(active))
             break;
@@ -181,6 +188,77 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
         }
         if (stop)
             break;
+    }
+}
+
+/* Resumable slice (serving): every unfinished pixel of every tile advances by at
+   most `budget` iterations; finished pixels write their count and decrement
+   their request's remaining counter, the others save z, z^2 and wz. */
+__global__ __launch_bounds__(256) void k_fp64_slice(const mdzg_dtile* tiles, int ntiles,
+                                                    uint64_t total,
+                                                    unsigned long long* counter, int budget)
+{
+    bool active = false;
+    int fractal = 0;
+    int32_t wz = 0, depth = 0, left = 0;
+    uint32_t local = 0;
+    const mdzg_dtile* tp = tiles;
+    fp64_state s;
+    s.wre = s.wim = s.wre2 = s.wim2 = s.cre = s.cim = 0.0;
+    bool drained = false;
+    for (;;) {
+        if (!drained) {
+          const bool need = !active;
+          bool any;
+          const uint64_t g = claim(need, counter, any);
+          if (need && g < total) {
+            const int ti = find_tile(tiles, ntiles, g);
+            tp = &tiles[ti];
+            local = (uint32_t)(g - tp->pix_begin);
+            if (tp->out[local] == 0xffffffffu) {
+                const int row = (int)(local / (uint32_t)tp->ncols);
+                const int col = (int)(local % (uint32_t)tp->ncols);
+                fp64_init(s, *tp, row, col);
+                wz = tp->wzs[local];
+                if (wz > 0) {
+                    const double* st = (const double*)tp->state + 4 * (size_t)local;
+                    s.wre = st[0];
+                    s.wim = st[1];
+                    s.wre2 = st[2];
+                    s.wim2 = st[3];
+                }
+                fractal = tp->fractal;
+                depth = tp->depth;
+                left = budget;
+                active = true;
+            }
+          }
+          drained = __any(need && g >= total);
+        }
+        if (!__any(active)) {
+            if (drained)
+                break;
+            continue;
+        }
+        for (int it = 0; it < 32; ++it) {
+            if (active) {
+                ++wz;
+                const bool esc = fp64_step(s, fractal, wz);
+                if (esc || wz >= depth) {
+                    tp->out[local] = esc ? (uint32_t)wz : 0u;
+                    atomicSub(tp->remaining, 1u);
+                    active = false;
+                } else if (--left == 0) {
+                    double* st = (double*)tp->state + 4 * (size_t)local;
+                    st[0] = s.wre;
+                    st[1] = s.wim;
+                    st[2] = s.wre2;
+                    st[3] = s.wim2;
+                    tp->wzs[local] = wz;
+                    active = false;
+                }
+            }
+        }
     }
 }
 
@@ -344,22 +422,27 @@ __global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_persist(c
     __shared__ uint32_t c_lds[c_in_lds<N>() ? 2 * (N + 2) * 256 : 1];
     rn_state<N> s;
     s.c.init(c_lds);
+    bool drained = false;
     for (;;) {
-        bool any;
-        const uint64_t g = claim(!active, counter, any);
-        if (!active && g < total) {
-            const int ti = find_tile(tiles, ntiles, g);
-            const mdzg_dtile& t = tiles[ti];
-            const uint32_t local = (uint32_t)(g - t.pix_begin);
-            const int row = (int)(local / (uint32_t)t.ncols);
-            const int col = (int)(local % (uint32_t)t.ncols);
-            rn_init<N, FULL>(s, t, row, col);
-            fractal = t.fractal;
-            kclr = t.kclr;
-            depth = t.depth;
-            out = t.out + local;
-            wz = 0;
-            active = true;
+        if (!drained) {
+            const bool need = !active;
+            bool any;
+            const uint64_t g = claim(need, counter, any);
+            if (need && g < total) {
+                const int ti = find_tile(tiles, ntiles, g);
+                const mdzg_dtile& t = tiles[ti];
+                const uint32_t local = (uint32_t)(g - t.pix_begin);
+                const int row = (int)(local / (uint32_t)t.ncols);
+                const int col = (int)(local % (uint32_t)t.ncols);
+                rn_init<N, FULL>(s, t, row, col);
+                fractal = t.fractal;
+                kclr = t.kclr;
+                depth = t.depth;
+                out = t.out + local;
+                wz = 0;
+                active = true;
+            }
+            drained = __any(need && g >= total);
         }
         if (!__any(active))
             break;
@@ -376,6 +459,94 @@ __global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_persist(c
         }
         if (stop)
             break;
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void store_mp(uint32_t* p, const mp<N>& v)
+{
+#pragma unroll
+    for (int j = 0; j < N; ++j) p[j] = v.m[j];
+    p[N] = (uint32_t)v.e;
+    p[N + 1] = v.s;
+}
+
+/* Resumable slice (serving), RN arithmetic: see k_fp64_slice. */
+template <int N, bool FULL>
+__global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_slice(
+    const mdzg_dtile* tiles, int ntiles, uint64_t total, unsigned long long* counter, int budget)
+{
+    bool active = false;
+    int fractal = 0, kclr = 0;
+    int32_t wz = 0, depth = 0, left = 0;
+    uint32_t local = 0;
+    const mdzg_dtile* tp = tiles;
+    __shared__ uint32_t c_lds[c_in_lds<N>() ? 2 * (N + 2) * 256 : 1];
+    rn_state<N> s;
+    s.c.init(c_lds);
+    bool drained = false;
+    for (;;) {
+        if (!drained) {
+          const bool need = !active;
+          bool any;
+          const uint64_t g = claim(need, counter, any);
+          if (need && g < total) {
+            const int ti = find_tile(tiles, ntiles, g);
+            tp = &tiles[ti];
+            local = (uint32_t)(g - tp->pix_begin);
+            if (tp->out[local] == 0xffffffffu) {
+                const int row = (int)(local / (uint32_t)tp->ncols);
+                const int col = (int)(local % (uint32_t)tp->ncols);
+                wz = tp->wzs[local];
+                if (wz == 0) {
+                    rn_init<N, FULL>(s, *tp, row, col);
+                } else {
+                    const uint32_t* st = tp->state + (size_t)local * 4 * MDZG_VS(N);
+                    load_mp<N>(s.wre, st);
+                    load_mp<N>(s.wim, st + MDZG_VS(N));
+                    load_mp<N>(s.wre2, st + 2 * MDZG_VS(N));
+                    load_mp<N>(s.wim2, st + 3 * MDZG_VS(N));
+                    mp<N> cr, ci;
+                    if (tp->family == 1) {
+                        load_mp<N>(cr, tp->jc);
+                        load_mp<N>(ci, tp->jc + MDZG_VS(N));
+                    } else {
+                        load_mp<N>(cr, tp->xs + (size_t)col * MDZG_VS(N));
+                        load_mp<N>(ci, tp->ys + (size_t)row * MDZG_VS(N));
+                    }
+                    s.c.put(cr, ci);
+                }
+                fractal = tp->fractal;
+                kclr = tp->kclr;
+                depth = tp->depth;
+                left = budget;
+                active = true;
+            }
+          }
+          drained = __any(need && g >= total);
+        }
+        if (!__any(active)) {
+            if (drained)
+                break;
+            continue;
+        }
+        if (active) {
+            ++wz;
+            const bool esc = rn_step<N, FULL>(s, fractal, wz, kclr);
+            if (esc || wz >= depth) {
+                tp->out[local] = esc ? (uint32_t)wz : 0u;
+                atomicSub(tp->remaining, 1u);
+                active = false;
+            } else if (--left == 0) {
+                uint32_t* st = tp->state + (size_t)local * 4 * MDZG_VS(N);
+                store_mp<N>(st, s.wre);
+                store_mp<N>(st + MDZG_VS(N), s.wim);
+                store_mp<N>(st + 2 * MDZG_VS(N), s.wre2);
+                store_mp<N>(st + 3 * MDZG_VS(N), s.wim2);
+                tp->wzs[local] = wz;
+                active = false;
+            }
+        }
     }
 }
 

@@ -10,34 +10,36 @@
 #include "mdzg_dev.h"
 
 extern "C" {
-void mdzg_kget_fp64(const void** tile, const void** persist);
-void mdzg_kget_rn1(int, const void**, const void**);
-void mdzg_kget_rn2(int, const void**, const void**);
-void mdzg_kget_rn3(int, const void**, const void**);
-void mdzg_kget_rn4(int, const void**, const void**);
-void mdzg_kget_rn6(int, const void**, const void**);
-void mdzg_kget_rn8(int, const void**, const void**);
-void mdzg_kget_rn12(int, const void**, const void**);
-void mdzg_kget_rn16(int, const void**, const void**);
-void mdzg_kget_rn24(int, const void**, const void**);
-void mdzg_kget_rn32(int, const void**, const void**);
+void mdzg_kget_fp64(const void** tile, const void** persist, const void** slice);
+void mdzg_kget_rn1(int, const void**, const void**, const void**);
+void mdzg_kget_rn2(int, const void**, const void**, const void**);
+void mdzg_kget_rn3(int, const void**, const void**, const void**);
+void mdzg_kget_rn4(int, const void**, const void**, const void**);
+void mdzg_kget_rn6(int, const void**, const void**, const void**);
+void mdzg_kget_rn8(int, const void**, const void**, const void**);
+void mdzg_kget_rn12(int, const void**, const void**, const void**);
+void mdzg_kget_rn16(int, const void**, const void**, const void**);
+void mdzg_kget_rn24(int, const void**, const void**, const void**);
+void mdzg_kget_rn32(int, const void**, const void**, const void**);
 }
 
-typedef void (*kget_fn)(int, const void**, const void**);
+typedef void (*kget_fn)(int, const void**, const void**, const void**);
 static const kget_fn kget_rn[MDZG_NCLASS_N] = {
     mdzg_kget_rn1, mdzg_kget_rn2, mdzg_kget_rn3, mdzg_kget_rn4, mdzg_kget_rn6,
     mdzg_kget_rn8, mdzg_kget_rn12, mdzg_kget_rn16, mdzg_kget_rn24, mdzg_kget_rn32,
 };
 
-static void lookup(int kid, int persist, const void** f)
+/* form: 0 = per-tile, 1 = persistent, 2 = resumable slice */
+static void lookup(int kid, int form, const void** f)
 {
     const void* t = nullptr;
     const void* p = nullptr;
+    const void* sl = nullptr;
     if (kid == 0)
-        mdzg_kget_fp64(&t, &p);
+        mdzg_kget_fp64(&t, &p, &sl);
     else
-        kget_rn[(kid - 1) >> 1]((kid - 1) & 1, &t, &p);
-    *f = persist ? p : t;
+        kget_rn[(kid - 1) >> 1]((kid - 1) & 1, &t, &p, &sl);
+    *f = form == 2 ? sl : (form == 1 ? p : t);
 }
 
 extern "C" {
@@ -77,6 +79,30 @@ int mdzg_dev_launch_persist(int kid, const mdzg_dtile* d_tiles, int ntiles, uint
     lookup(kid, 1, &f);
     void* args[] = { (void*)&d_tiles, (void*)&ntiles, (void*)&total, (void*)&d_counter,
                      (void*)&d_cancel, (void*)&chunk };
+    return hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, st) == hipSuccess ? 0 : -3;
+}
+
+int mdzg_dev_occupancy_slice(int kid)
+{
+    if (kid < 0 || kid >= mdzg_dev_nkernels())
+        return 0;
+    const void* f;
+    lookup(kid, 2, &f);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, 0) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+int mdzg_dev_launch_slice(int kid, const mdzg_dtile* d_tiles, int ntiles, uint64_t total,
+                          unsigned long long* d_counter, int grid, int budget, hipStream_t st)
+{
+    if (kid < 0 || kid >= mdzg_dev_nkernels() || ntiles <= 0 || grid <= 0 || budget <= 0)
+        return -1;
+    const void* f;
+    lookup(kid, 2, &f);
+    void* args[] = { (void*)&d_tiles, (void*)&ntiles, (void*)&total, (void*)&d_counter,
+                     (void*)&budget };
     return hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, st) == hipSuccess ? 0 : -3;
 }
 

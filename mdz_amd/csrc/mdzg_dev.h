@@ -37,6 +37,11 @@ typedef struct mdzg_dtile {
     int32_t kclr;               /* RN: 32N - p bits cleared by rounding              */
     int32_t depth;              /* max iterations (image_info.depth)                 */
     int32_t vs;                 /* RN: words per stored value                        */
+    int32_t pad0;
+    /* resumable slices (serving): per-pixel orbit state between slices */
+    uint32_t* state;            /* z, z^2 per pixel: RN 4 stored values, FP64 4 doubles */
+    int32_t* wzs;               /* iterations done per pixel (0 = not started)       */
+    unsigned int* remaining;    /* unfinished pixels of this request                 */
 } mdzg_dtile;
 
 #endif

@@ -22,6 +22,9 @@ int mdzg_dev_launch_tile(int kid, const mdzg_dtile* d_tile, uint32_t npix, hipSt
 int mdzg_dev_launch_persist(int kid, const mdzg_dtile* d_tiles, int ntiles, uint64_t total,
                             unsigned long long* d_counter, const int* d_cancel, int grid,
                             int chunk, hipStream_t st);
+int mdzg_dev_occupancy_slice(int kid);
+int mdzg_dev_launch_slice(int kid, const mdzg_dtile* d_tiles, int ntiles, uint64_t total,
+                          unsigned long long* d_counter, int grid, int budget, hipStream_t st);
 
 extern const int mdzg_nwords[MDZG_NCLASS_N];
 

@@ -1,0 +1,100 @@
+"""GPU tests of the continuous-batching server: each request's counts are the
+same alone, in a mix, across many slices (orbit state saved and resumed in
+HBM), under a tight HBM budget, and in the per-request baseline form."""
+import numpy as np
+import pytest
+
+import mdz_amd as M
+from mdz_amd import workloads as WL
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _eq(a, b):
+    return np.array_equal(np.asarray(a).astype(np.int64), np.asarray(b).astype(np.int64))
+
+
+def _mix():
+    """Requests across every arithmetic: MPFR at aligned and ragged p, long
+    double, FP64, all four formulas, Julia."""
+    reqs = []
+    for name in ("test", "subdudehue", "satellite", "honeytrace", "spiralblues", "confirm",
+                 "mdz_banner", "floral"):
+        rect, p, depth, fractal, path, (W, H) = cases.scene_rect(name)
+        v = M.make_view(rect, precision=p, depth=min(depth, 3000), width=W, height=H,
+                        fractal=fractal, path=path)
+        reqs.append((v, (H // 3, min(12, H // 3)), (W // 5, 48)))
+    v1 = WL.config1_view(fractal=2)
+    reqs.append((v1, (30, 40), (0, 240)))
+    J = cases.JULIA
+    for fr in range(4):
+        vj = M.make_view(J["rect"], precision=96, depth=800, width=96, height=72, fractal=fr,
+                         family=M.FAMILY_JULIA, julia=J["c"])
+        reqs.append((vj, (10, 20), (0, 96)))
+    return reqs
+
+
+@pytest.mark.parametrize("slice_iters", [0, 7])
+def test_server_results_equal_standalone(ctx, slice_iters):
+    reqs = _mix()
+    alone = [ctx.render(v, lines=l, cols=c)[0] for v, l, c in reqs]
+    with M.Server(ctx, slice_iters=slice_iters) as srv:
+        ids = [srv.submit(v, l, c) for v, l, c in reqs]
+        outs = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    assert st["completed"] == len(reqs)
+    if slice_iters == 7:
+        assert st["slices"] > 100           # many save/resume cycles per pixel
+    for (out, info), ref in zip(outs, alone):
+        assert info["status"] == 2 and info["latency_ms"] >= 0
+        assert _eq(out, ref)
+
+
+def test_server_tight_hbm_budget(ctx):
+    """Only a few requests fit at once: admission queues the rest, results hold."""
+    reqs = _mix() * 2
+    alone = [ctx.render(v, lines=l, cols=c)[0] for v, l, c in reqs]
+    budget = 4 << 20
+    with M.Server(ctx, hbm_budget=budget, slice_iters=64) as srv:
+        ids = [srv.submit(v, l, c) for v, l, c in reqs]
+        outs = [srv.wait(i)[0] for i in ids]
+        st = srv.stats()
+    assert st["hbm_peak"] <= budget
+    for out, ref in zip(outs, alone):
+        assert _eq(out, ref)
+
+
+def test_server_per_request_mode(ctx):
+    reqs = _mix()
+    alone = [ctx.render(v, lines=l, cols=c)[0] for v, l, c in reqs]
+    with M.Server(ctx, mode=M.SERVE_PER_REQUEST) as srv:
+        ids = [srv.submit(v, l, c) for v, l, c in reqs]
+        outs = [srv.wait(i)[0] for i in ids]
+    for out, ref in zip(outs, alone):
+        assert _eq(out, ref)
+
+
+def test_server_rejects_request_larger_than_budget(ctx):
+    v = WL.config2_view()
+    with M.Server(ctx, hbm_budget=1 << 20) as srv:
+        with pytest.raises(M.MdzgError) as e:
+            srv.submit(v)
+        assert e.value.code == -5
+
+
+def test_server_arrivals_while_running(ctx):
+    """Requests submitted while earlier ones are on the device join and finish."""
+    v = WL.config2_view()
+    import time
+    with M.Server(ctx, slice_iters=256) as srv:
+        first = srv.submit(v)                     # 1.5e9 pixel-iterations
+        time.sleep(0.002)
+        small = [srv.submit(v, (8 * k, 8), (0, 64)) for k in range(10)]
+        outs = [srv.wait(i) for i in small]
+        big, info_big = srv.wait(first)
+    ref = ctx.render(v)[0]
+    assert _eq(big, ref)
+    for k, (o, info) in enumerate(outs):
+        assert _eq(o, ref[8 * k:8 * k + 8, :64])
+        assert info["admit_ms"] < info_big["done_ms"]     # admitted while the big one ran
