@@ -101,6 +101,74 @@ def config3_batched(ctx, M, WL, np, ntiles=256):
                      identical_counts=bool(same)))
 
 
+def crop_tiles(WL, n, seed=0x5E):
+    """64x64 crops of the config 2 view at seeded positions (p=128, depth 5000)."""
+    import random
+    v = WL.config2_view()
+    rng = random.Random(seed)
+    return [(v, (rng.randrange(0, 480 - 64), 64), (rng.randrange(0, 640 - 64), 64))
+            for _ in range(n)], v
+
+
+def serve_run(ctx, M, np, mode, tiles, rate, seed=1):
+    """Poisson arrivals at `rate` requests/s (inf: all at once); latency per request
+    is done - submit on the host clock; throughput over first submit .. last done."""
+    import random
+    rng = random.Random(seed)
+    srv = M.Server(ctx, mode=mode)
+    t0 = time.perf_counter()
+    tnext = 0.0
+    ids = []
+    for v, l, c in tiles:
+        if rate != float("inf"):
+            tnext += rng.expovariate(rate)
+            dt = tnext - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+        ids.append(srv.submit(v, l, c))
+    res = [srv.wait(i)[1] for i in ids]
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r["latency_ms"] for r in res])
+    span = (max(r["done_ms"] for r in res) - min(r["submit_ms"] for r in res)) * 1e-3
+    pix = sum(r["pixel_iters"] for r in res)
+    return dict(rate=rate, n=len(tiles), p50_ms=float(np.percentile(lat, 50)),
+                p99_ms=float(np.percentile(lat, 99)), mean_ms=float(lat.mean()),
+                tiles_per_s=len(tiles) / span, gpix_per_s=pix / span / 1e9,
+                slices=st["slices"], hbm_peak=st["hbm_peak"])
+
+
+def unloaded_latency(ctx, M, np, mode, tiles, reps=15):
+    srv = M.Server(ctx, mode=mode)
+    lat = []
+    for v, l, c in tiles[:reps]:
+        i = srv.submit(v, l, c)
+        lat.append(srv.wait(i)[1]["latency_ms"])
+    srv.close()
+    return float(np.median(lat))
+
+
+def config4_serving(ctx, M, WL, np, fracs=(0.25, 0.5, 0.75, 0.9), n=300):
+    """Config 4: Poisson tile requests swept towards saturation, continuous
+    batching against one launch per request."""
+    tiles, _ = crop_tiles(WL, n)
+    out = {}
+    for name, mode in (("batched", M.SERVE_BATCHED), ("per_request", M.SERVE_PER_REQUEST)):
+        unl = unloaded_latency(ctx, M, np, mode, tiles)
+        sat = serve_run(ctx, M, np, mode, tiles, float("inf"))
+        log(f"serve {name}: unloaded {unl:.2f} ms, saturation {sat['tiles_per_s']:.0f} tiles/s "
+            f"{sat['gpix_per_s']:.1f} G pixel-iter/s")
+        sweep = []
+        for f in fracs:
+            r = serve_run(ctx, M, np, mode, tiles, f * sat["tiles_per_s"])
+            r["load"] = f
+            sweep.append(r)
+            log(f"  load {f:.2f}: {r['tiles_per_s']:.0f} tiles/s p50 {r['p50_ms']:.2f} "
+                f"p99 {r['p99_ms']:.2f} ms")
+        out[name] = dict(unloaded_ms=unl, saturation=sat, sweep=sweep)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +248,21 @@ def main():
                              / 1e12 / PEAK_TOPS)
         log(f"rn1024: {res['rn1024']['kernel_gpix_per_s']:.3f} G pixel-iter/s (kernel)")
 
+        # p50/p99 tile latency: continuous batching, 64x64 config 2 crops
+        tiles, _ = crop_tiles(WL, 300)
+        unl = unloaded_latency(ctx, M, np, M.SERVE_BATCHED, tiles)
+        sat = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, float("inf"))
+        half = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, 0.5 * sat["tiles_per_s"])
+        res["latency"] = dict(workload="Poisson requests of 64x64 crops of config 2 (p=128, "
+                              "depth 5000), continuous batching", unloaded_p50_ms=unl,
+                              saturation_tiles_per_s=sat["tiles_per_s"],
+                              saturation_gpix_per_s=sat["gpix_per_s"],
+                              at_half_load=dict(rate=half["rate"], p50_ms=half["p50_ms"],
+                                                p99_ms=half["p99_ms"],
+                                                tiles_per_s=half["tiles_per_s"]))
+        log(f"latency: unloaded {unl:.2f} ms; half load p50 {half['p50_ms']:.2f} "
+            f"p99 {half['p99_ms']:.2f} ms")
+
         if not a.no_cpu_baseline:
             log("cpu baseline (MDZ MPFR loop on host cores) ...")
             rect = [s.decode() for s in v._keep[:4]]
@@ -187,6 +270,16 @@ def main():
             log(f"cpu baseline: {res['cpu_baseline']['value']:.4f} G pixel-iter/s")
     elif a.config == "3":
         res.update(config3_batched(ctx, M, WL, np))
+    elif a.config == "4":
+        sv = config4_serving(ctx, M, WL, np)
+        b, t = sv["batched"], sv["per_request"]
+        res.update(value=b["saturation"]["gpix_per_s"], steps=1, warmup=0,
+                   ms_per_step=None, dtype="u32 limbs, RN p=128",
+                   config=dict(workload="config4 Poisson requests of 64x64 crops of the config 2 "
+                               "view (p=128, depth 5000), swept to saturation"),
+                   serving=sv,
+                   throughput_ratio_batched_vs_per_request=b["saturation"]["gpix_per_s"] /
+                   t["saturation"]["gpix_per_s"])
     else:
         raise SystemExit(f"unknown --config {a.config}")
 
