@@ -169,6 +169,67 @@ def config4_serving(ctx, M, WL, np, fracs=(0.25, 0.5, 0.75, 0.9), n=300):
     return out
 
 
+def config5_deep(ctx, M, WL, np, cpu=True):
+    """Config 5: the whole 3840x2160-sample frame at p=1024 through the
+    line-callback adapter (16 worker threads claiming rows, as MDZ's render
+    threads do), then MDZ's MPFR loop on a seeded 64x64 crop on the host."""
+    import threading
+    v = WL.config5_view()
+    W, H = v.real_width, v.real_height
+    raw = np.full((H, W), -1, dtype=np.int32)
+    fr = M.Frame(ctx, v, raw)
+    nxt = [0]
+    lock = threading.Lock()
+    t0 = time.perf_counter()
+
+    def worker():
+        while True:
+            with lock:
+                line = nxt[0]
+                nxt[0] += 1
+                if line % 216 == 0:
+                    log(f"config 5: row {line} at {time.perf_counter() - t0:.1f} s")
+            if line >= H or fr.calculate_line(line) != 1:
+                return
+
+    ths = [threading.Thread(target=worker) for _ in range(16)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    el = time.perf_counter() - t0
+    rc, st = fr.close()
+    pi = float(np.where(raw > 0, raw, 20000).sum())
+    out = dict(workload="config5 Mandelbrot 1920x1080 x 2x2 AA = 3840x2160 samples, p=1024, "
+               "width 1e-290 at (-0.75, 0), depth 20000, via the line-callback adapter",
+               rc=rc, seconds=el, pixel_iters=pi, gpix_per_s=pi / el / 1e9,
+               sum_band_kernel_ms=st["kernel_ms"],
+               frac_peak=2 * 2080 * pi / el / 1e12 / PEAK_TOPS,
+               all_inside=bool((raw == 0).all()))
+    if cpu:
+        import random
+        from oracle import oracle as O
+        rng = random.Random(5)
+        l0, c0 = rng.randrange(H - 64), rng.randrange(W - 64)
+        rect = [s_.decode() for s_ in v._keep[:4]]
+        try:
+            O.use_reference(True)
+            kind = "reference"
+        except Exception:
+            kind = "port"
+        counts, cel = O.render(rect, precision=1024, depth=20000, width=W, height=H,
+                               lines=(l0, 64, 1), cols=(c0, 64), threads=host_cores(),
+                               timed=True)
+        O.use_reference(False)
+        cpi = float(np.where(counts > 0, counts, 20000).sum())
+        out["cpu_baseline"] = dict(value=cpi / cel / 1e9, unit="G pixel-iter/s",
+                                   cores=host_cores(), kind=kind,
+                                   sample=f"64x64 crop at row {l0} col {c0}: {cpi:.3e} "
+                                          f"pixel-iterations in {cel:.2f} s")
+        out["crop_matches_gpu"] = bool(np.array_equal(counts, raw[l0:l0 + 64, c0:c0 + 64]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,6 +331,12 @@ def main():
             log(f"cpu baseline: {res['cpu_baseline']['value']:.4f} G pixel-iter/s")
     elif a.config == "3":
         res.update(config3_batched(ctx, M, WL, np))
+    elif a.config == "5":
+        r = config5_deep(ctx, M, WL, np, cpu=not a.no_cpu_baseline)
+        res.update(value=r["gpix_per_s"], steps=1, warmup=0, ms_per_step=1e3 * r["seconds"],
+                   dtype="u32 limbs, RN p=1024", config=dict(workload=r["workload"]), deep=r)
+        if "cpu_baseline" in r:
+            res["cpu_baseline"] = r["cpu_baseline"]
     elif a.config == "4":
         sv = config4_serving(ctx, M, WL, np)
         b, t = sv["batched"], sv["per_request"]
