@@ -30,6 +30,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "G pixel-iter/s at FP64/128/1024-bit vs MPFR CPU; p50/p99 tile latency"
 PEAK_TOPS = 78.6
+# HBM bytes per config 2 launch of k_rn_persist<4,true>: rocprofv3 FETCH_SIZE x2 +
+# WRITE_SIZE x1 (both scaled by a 256 MiB copy's readings); the I/O of the
+# kernel does not depend on its arithmetic, so it is measured once
+CONFIG2_TRAFFIC_BYTES = 3.07e6
 
 
 def log(*a):
@@ -237,12 +241,28 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the config 2 timed steps (for counter runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only a plain device-to-device copy of MIB MiB (calibrates "
+                         "FETCH_SIZE / WRITE_SIZE against known bytes)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the escape-time tiles shard across GPUs with no "
               "exchange, and this build runs on one device only (use --gpus 1).",
               file=sys.stderr)
         sys.exit(2)
+
+    if a.copy_probe:
+        import torch
+        n = a.copy_probe << 18
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(copy_probe_bytes=4 * n, launches=3)))
+        return
 
     import numpy as np
     import mdz_amd as M
@@ -277,9 +297,14 @@ def main():
                                pixel_iters_per_step=pi_step, kernel="k_rn_persist<4,true>"),
                    kernel_gpix_per_s=pi_step / (kavg * 1e-3) / 1e9,
                    roofline=dict(bound="valu", achieved=achieved, peak=PEAK_TOPS, unit="TOP/s",
-                                 frac=achieved / PEAK_TOPS, traffic=None,
+                                 frac=achieved / PEAK_TOPS, traffic=CONFIG2_TRAFFIC_BYTES,
                                  kernel_ms=kavg,
                                  ops="36 MAC (x2 ops) per pixel-iteration x 1.536e9 per launch"))
+
+        if a.headline_only:
+            ctx.close()
+            print(json.dumps(res))
+            return
 
         # FP64 figure: config 1 frame
         v1 = WL.config1_view()
