@@ -284,6 +284,15 @@ constexpr int rn_min_waves()
     return (N >= 32 && !FULL) ? 1 : (N >= 12 ? 2 : 4);
 }
 
+/* the slice kernel keeps its resume and save paths live inside the
+   iteration loop, so it gets a larger share of the register file than the
+   other kernels from 8 words up (a single wave's whole file from 24) */
+template <int N, bool FULL>
+constexpr int rn_min_waves_slice()
+{
+    return N >= 24 ? 1 : (N >= 8 ? 2 : rn_min_waves<N, FULL>());
+}
+
 /* c per lane: registers for small N; for N >= 24 the block's LDS (word w of
    lane t at [w][t], conflict-free), so the register file holds z and z^2 only
    and two waves fit per SIMD */
@@ -473,7 +482,7 @@ __device__ __forceinline__ void store_mp(uint32_t* p, const mp<N>& v)
 
 /* Resumable slice (serving), RN arithmetic: see k_fp64_slice. */
 template <int N, bool FULL>
-__global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_slice(
+__global__ __launch_bounds__(256, (rn_min_waves_slice<N, FULL>())) void k_rn_slice(
     const mdzg_dtile* tiles, int ntiles, uint64_t total, unsigned long long* counter, int budget)
 {
     bool active = false;

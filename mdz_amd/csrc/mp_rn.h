@@ -29,7 +29,7 @@
  * device form of mac96 below; the host form computes the same function in
  * portable C).  Word carry chains use v_add_co/v_addc_co via __builtin_addc.
  * Rare cases (carry through a rounding increment, cancellation of a whole
- * word, a borrow that flips the sign) run under wave-wide votes, so a wave
+ * word, an alignment shift of whole words) run under wave-wide votes, so a wave
  * only pays for them when one of its lanes needs them.  Built for the host
  * (tests only) the vote is the lane's own predicate.
  */
@@ -261,6 +261,23 @@ MP_HD void acc96_add(uint64_t& acc, uint32_t& c2, uint64_t accb, uint32_t c2b)
     acc = ((uint64_t)l1 << 32) | l0;
 }
 
+/* first product of a column: the third accumulator word starts from the carry */
+MP_HD void mac96_first(uint32_t a, uint32_t b, uint64_t& acc, uint32_t& c2)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t cc;
+    asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
+        "v_addc_co_u32 %2, %1, 0, 0, %1"
+        : "+v"(acc), "=&s"(cc), "=v"(c2)
+        : "v"(a), "v"(b));
+#else
+    const uint64_t p = (uint64_t)a * b;
+    const uint64_t t = acc + p;
+    c2 = (uint32_t)(t < p);
+    acc = t;
+#endif
+}
+
 template <int N>
 MP_HD void mant_mul(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t& S)
 {
@@ -274,14 +291,17 @@ MP_HD void mant_mul(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t&
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             xa[q] = q == 0 ? acc : 0;
-            xc[q] = q == 0 ? c2 : 0;
+            xc[q] = 0;       /* set by the chain's first multiply-accumulate */
         }
         int n = 0;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             const int j = k - i;
             if (j >= 0 && j < N) {
-                mac96(a[i], b[j], xa[n % K], xc[n % K]);
+                if (n < K)   /* first product of chain n */
+                    mac96_first(a[i], b[j], xa[n % K], xc[n % K]);
+                else
+                    mac96(a[i], b[j], xa[n % K], xc[n % K]);
                 ++n;
             }
         }
@@ -295,7 +315,6 @@ MP_HD void mant_mul(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t&
         else
             H[k - (N - 1)] = (uint32_t)acc;
         acc = (acc >> 32) | ((uint64_t)c2 << 32);
-        c2 = 0;
     }
     H[N] = (uint32_t)acc;
     S = (uint32_t)(st != 0);
@@ -366,13 +385,16 @@ MP_HD void mul_rn(mp<N>& r, const mp<N>& a, const mp<N>& b, int kclr)
         mant_mul<N>(a.m, SQR ? a.m : b.m, H, S);
     int32_t e = SQR ? (a.e + a.e) : (a.e + b.e);
     const uint32_t s = SQR ? 0u : (a.s ^ b.s);
-    /* product of normalised values is >= 2^(64N-2): at most one left shift */
+    /* product of normalised values is >= 2^(64N-2): at most one left shift,
+       skipped by a wave none of whose lanes needs it */
     const bool sh = (H[N] >> 31) == 0;
+    if (wave_any(sh)) {
 #pragma unroll
-    for (int j = N; j >= 1; --j)
-        H[j] = sh ? funnel_r(H[j], H[j - 1], 31u) : H[j];
-    H[0] = sh ? (H[0] << 1) : H[0];
-    e -= sh ? 1 : 0;
+        for (int j = N; j >= 1; --j)
+            H[j] = sh ? funnel_r(H[j], H[j - 1], 31u) : H[j];
+        H[0] = sh ? (H[0] << 1) : H[0];
+        e -= sh ? 1 : 0;
+    }
     round_p<N>(H + 1, H[0], S, e, kclr);
     const bool z = (H[N] == 0);
 #pragma unroll
@@ -387,20 +409,28 @@ MP_HD void mul_rn(mp<N>& r, const mp<N>& a, const mp<N>& b, int kclr)
 template <int N>
 MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb, int kclr)
 {
-    /* order by exponent: A gets the larger one */
-    const bool sw = b_in.e > a_in.e;
-    const uint64_t swm = lane_mask(sw);
+    /* order by exponent: A gets the larger one.  Lanes of a wave walk
+       neighbouring pixels whose orbits mostly agree, so a step that no lane
+       needs (here: any swap at all) is skipped for the whole wave. */
+    const uint64_t swm = lane_mask(b_in.e > a_in.e);
     uint32_t A[N], B[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        A[j] = sel_mask(swm, a_in.m[j], b_in.m[j]);
-        B[j] = sel_mask(swm, b_in.m[j], a_in.m[j]);
+        A[j] = a_in.m[j];
+        B[j] = b_in.m[j];
+    }
+    if (swm != 0) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            A[j] = sel_mask(swm, a_in.m[j], b_in.m[j]);
+            B[j] = sel_mask(swm, b_in.m[j], a_in.m[j]);
+        }
     }
     const uint32_t sbin = b_in.s ^ flipb;
-    const uint32_t sa = sw ? sbin : a_in.s;
-    const uint32_t sb = sw ? a_in.s : sbin;
-    int32_t e = sw ? b_in.e : a_in.e;
-    const int32_t eb = sw ? a_in.e : b_in.e;
+    const uint32_t sa = sel_mask(swm, a_in.s, sbin);
+    const uint32_t sb = sel_mask(swm, sbin, a_in.s);
+    int32_t e = (int32_t)sel_mask(swm, (uint32_t)a_in.e, (uint32_t)b_in.e);
+    const int32_t eb = (int32_t)sel_mask(swm, (uint32_t)b_in.e, (uint32_t)a_in.e);
 
     /* align B right by d bits into N+1 words X (X[0] = guard word) + sticky */
     uint32_t d = (uint32_t)(e - eb);
@@ -439,7 +469,6 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
     T[0] = add_cc(X[0] ^ msk, 0u, c, c);
 #pragma unroll
     for (int j = 1; j <= N; ++j) T[j] = add_cc(A[j - 1], X[j] ^ msk, c, c);
-    uint32_t s = sa;
 
     /* same signs with a carry out: shift right one bit */
     const bool rs = (sub == 0) & (c != 0);
@@ -452,16 +481,18 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
     }
 
     /* a borrow (only possible when d == 0, so S == 0): negate, flip sign */
+    uint32_t sres = sa;
     const bool ng = (sub != 0) & (c == 0);
     if (wave_any(ng)) {
         uint32_t cc = ng ? 1u : 0u;
         const uint32_t nm = ng ? 0xffffffffu : 0u;
 #pragma unroll
         for (int j = 0; j <= N; ++j) T[j] = add_cc(T[j] ^ nm, 0u, cc, cc);
-        s = ng ? (s ^ 1u) : s;
+        sres = ng ? (sres ^ 1u) : sres;
     }
 
-    /* cancellation of the whole top word (exact case, S == 0): word shift */
+    /* cancellation of the whole top word (a subtraction with d <= 1, which
+       is exact: S == 0): word shift */
     bool zero = false;
     if (wave_any(T[N] == 0)) {
         uint32_t lzw = 0;
@@ -496,11 +527,13 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
         e -= (int32_t)lz;
     }
 
+    /* an exact zero leaves T all zero words through the shifts and the
+       rounding (no guard, no sticky), so only exponent and sign change */
     round_p<N>(T + 1, T[0], S, e, kclr);
 #pragma unroll
-    for (int j = 0; j < N; ++j) r.m[j] = zero ? 0u : T[j + 1];
+    for (int j = 0; j < N; ++j) r.m[j] = T[j + 1];
     r.e = zero ? EZERO : e;
-    r.s = zero ? 0u : s;
+    r.s = zero ? 0u : sres;
 }
 
 /* r = 2a (exact) */

@@ -108,6 +108,11 @@ static void near_op(mp<N>& b, const mp<N>& a, int p)
         return;
     }
     b = a;
+    if (rnd32() % 8 == 0) {
+        // exact copy: a - a cancels to zero, a + a doubles
+        b.s = rnd32() & 1u;
+        return;
+    }
     int kclr = 32 * N - p;
     int nb = 1 + (int)(rnd32() % (uint32_t)(p < 70 ? p : 70));
     for (int i = 0; i < 3; ++i) {
