@@ -60,7 +60,8 @@ struct mp {
 MP_HD bool wave_any(bool c)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __any(c) != 0;
+    /* the vote's lane mask is the compare's own scalar result: no VALU work */
+    return __builtin_amdgcn_ballot_w64(c) != 0;
 #else
     return c;
 #endif
@@ -70,7 +71,7 @@ MP_HD bool wave_any(bool c)
 MP_HD uint64_t lane_mask(bool c)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __ballot(c);
+    return __builtin_amdgcn_ballot_w64(c);
 #else
     return c ? 1u : 0u;
 #endif
