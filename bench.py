@@ -34,6 +34,11 @@ PEAK_TOPS = 78.6
 # WRITE_SIZE x1 (both scaled by a 256 MiB copy's readings); the I/O of the
 # kernel does not depend on its arithmetic, so it is measured once
 CONFIG2_TRAFFIC_BYTES = 3.07e6
+# VALU instructions (wave64) per config 2 launch of k_rn_persist<4,true>:
+# rocprofv3 SQ_INSTS_VALU of the current kernel build
+CONFIG2_VALU_INSTS = 9.3233e9
+# VALU issue limit: 256 CUs x one wave64 instruction per clock x 2.4 GHz peak clock
+PEAK_VALU_ISSUE = 256 * 2.4e9
 
 
 def log(*a):
@@ -299,7 +304,12 @@ def main():
                    roofline=dict(bound="valu", achieved=achieved, peak=PEAK_TOPS, unit="TOP/s",
                                  frac=achieved / PEAK_TOPS, traffic=CONFIG2_TRAFFIC_BYTES,
                                  kernel_ms=kavg,
-                                 ops="36 MAC (x2 ops) per pixel-iteration x 1.536e9 per launch"))
+                                 ops="36 MAC (x2 ops) per pixel-iteration x 1.536e9 per launch",
+                                 valu_issue=dict(
+                                     insts_per_launch=CONFIG2_VALU_INSTS,
+                                     achieved=CONFIG2_VALU_INSTS / (kavg * 1e-3),
+                                     peak=PEAK_VALU_ISSUE, unit="wave64 VALU instructions/s",
+                                     frac=CONFIG2_VALU_INSTS / (kavg * 1e-3) / PEAK_VALU_ISSUE)))
 
         if a.headline_only:
             ctx.close()
