@@ -279,14 +279,16 @@ MP_HD void mac96_first(uint32_t a, uint32_t b, uint64_t& acc, uint32_t& c2)
 #endif
 }
 
-template <int N>
+/* K0 > 0 leaves out the columns below K0 (a short product, see mul_rn);
+   S is then meaningless */
+template <int N, int K0>
 MP_HD void mant_mul(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t& S)
 {
     constexpr int K = mac_chains<N>();
     uint64_t acc = 0;
     uint32_t c2 = 0, st = 0;
 #pragma unroll
-    for (int k = 0; k < 2 * N - 1; ++k) {
+    for (int k = K0; k < 2 * N - 1; ++k) {
         uint64_t xa[K];
         uint32_t xc[K];
 #pragma unroll
@@ -323,14 +325,14 @@ MP_HD void mant_mul(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t&
 
 /* Square with the off-diagonal products formed once and doubled per column
    (n(n+1)/2 multiply-accumulates); pays off from N >= 24 words. */
-template <int N>
+template <int N, int K0>
 MP_HD void mant_sqr_sym(const uint32_t* a, uint32_t* H, uint32_t& S)
 {
     constexpr int K = mac_chains<N>();
     uint64_t acc = 0;
     uint32_t c2 = 0, st = 0;
 #pragma unroll
-    for (int k = 0; k < 2 * N - 1; ++k) {
+    for (int k = K0; k < 2 * N - 1; ++k) {
         uint64_t xa[K];
         uint32_t xc[K];
 #pragma unroll
@@ -374,20 +376,49 @@ MP_HD void mant_sqr_sym(const uint32_t* a, uint32_t* H, uint32_t& S)
     S = (uint32_t)(st != 0);
 }
 
-/* r = RN_p(a * b) (SQR: b ignored, a*a) */
-template <int N, bool SQR>
-MP_HD void mul_rn(mp<N>& r, const mp<N>& a, const mp<N>& b, int kclr)
+/* Short products.  From MDZG_SHORT_MIN words up, mul_rn first forms only
+   the columns k >= K0 = N - 2 of the 2N-word product.  The columns left out
+   sum to D < K0 * 2^(32(K0+1)) (each column k holds k+1 products below
+   2^64), i.e. less than N - 2 units of the guard word H[0], so the exact
+   product lies in [T, T + (N-1) units of H[0]) where T is the short
+   product with its discarded column-K0 word; after the one-bit
+   normalisation, within 2(N-1) units.  Round-to-nearest of every value in
+   that interval is the same unless the interval reaches a midpoint:
+   for p = 32N the midpoint is H[0] = 2^31, for ragged p it lies on a word
+   boundary of H[0] (H[0] = 0 with the kept part's low bits 10..0, or a
+   carry out of H[0] into low bits 01..1).  Those lanes (a few in 2^32) take
+   the full product; the others round T with a nonzero sticky bit, which is
+   exact because the exact value is not a midpoint.  MPFR's own mulhigh
+   path rests on the same bound-and-check argument. */
+#ifndef MDZG_SHORT_MIN
+#define MDZG_SHORT_MIN 6
+#endif
+#if defined(MDZG_COUNT_SHORT_FALLBACK) && !defined(__HIP_DEVICE_COMPILE__)
+extern long mdzg_short_fallbacks;   /* host tests: how often the check failed */
+#define MDZG_NOTE_SHORT_FALLBACK() (++mdzg_short_fallbacks)
+#else
+#define MDZG_NOTE_SHORT_FALLBACK() ((void)0)
+#endif
+template <int N>
+constexpr int short_col()
 {
-    uint32_t H[N + 1];
-    uint32_t S;
+    return N >= MDZG_SHORT_MIN ? N - 2 : 0;
+}
+
+template <int N, bool SQR, int K0>
+MP_HD void mant_product(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t& S)
+{
     if (SQR && N >= 24)
-        mant_sqr_sym<N>(a.m, H, S);
+        mant_sqr_sym<N, K0>(a, H, S);
     else
-        mant_mul<N>(a.m, SQR ? a.m : b.m, H, S);
-    int32_t e = SQR ? (a.e + a.e) : (a.e + b.e);
-    const uint32_t s = SQR ? 0u : (a.s ^ b.s);
-    /* product of normalised values is >= 2^(64N-2): at most one left shift,
-       skipped by a wave none of whose lanes needs it */
+        mant_mul<N, K0>(a, SQR ? a : b, H, S);
+}
+
+/* product of normalised values is >= 2^(64N-2): at most one left shift,
+   skipped by a wave none of whose lanes needs it */
+template <int N>
+MP_HD void norm_product(uint32_t* H, int32_t& e)
+{
     const bool sh = (H[N] >> 31) == 0;
     if (wave_any(sh)) {
 #pragma unroll
@@ -395,6 +426,32 @@ MP_HD void mul_rn(mp<N>& r, const mp<N>& a, const mp<N>& b, int kclr)
             H[j] = sh ? funnel_r(H[j], H[j - 1], 31u) : H[j];
         H[0] = sh ? (H[0] << 1) : H[0];
         e -= sh ? 1 : 0;
+    }
+}
+
+/* r = RN_p(a * b) (SQR: b ignored, a*a) */
+template <int N, bool SQR>
+MP_HD void mul_rn(mp<N>& r, const mp<N>& a, const mp<N>& b, int kclr)
+{
+    constexpr int K0 = short_col<N>();
+    uint32_t H[N + 1];
+    uint32_t S;
+    mant_product<N, SQR, K0>(a.m, b.m, H, S);
+    const int32_t e0 = SQR ? (a.e + a.e) : (a.e + b.e);
+    int32_t e = e0;
+    const uint32_t s = SQR ? 0u : (a.s ^ b.s);
+    norm_product<N>(H, e);
+    if (K0 > 0) {
+        constexpr uint32_t U = 2u * N;
+        const bool unsure = (kclr == 0) ? ((uint32_t)(0x80000000u - H[0]) < U)
+                                        : ((uint32_t)(H[0] + (U - 1u)) < U);
+        S = 1;
+        if (wave_any(unsure)) {
+            MDZG_NOTE_SHORT_FALLBACK();
+            mant_product<N, SQR, 0>(a.m, b.m, H, S);
+            e = e0;
+            norm_product<N>(H, e);
+        }
     }
     round_p<N>(H + 1, H[0], S, e, kclr);
     const bool z = (H[N] == 0);

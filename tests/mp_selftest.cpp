@@ -15,7 +15,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#define MDZG_COUNT_SHORT_FALLBACK 1
 #include "../mdz_amd/csrc/mp_rn.h"
+
+long mdzg::mdzg_short_fallbacks = 0;
 
 using namespace mdzg;
 
@@ -90,6 +93,12 @@ static void rand_op(mp<N>& a, int p, int emin, int emax)
         for (int j = 0; j < N; ++j) a.m[j] = 0xffffffffu;
     } else if (kind == 2) {     // sparse: few bits
         for (int j = 0; j < N; ++j) a.m[j] = (rnd32() % 4 == 0) ? (1u << (rnd32() % 32)) : 0u;
+    } else if (kind == 4 || kind == 5) {
+        // top and bottom words only; kind 5 with top word 2^31: products of
+        // two such values put 0 or 2^31 in the guard word (short-product
+        // checks of mul_rn fall back to the full product there)
+        for (int j = 1; j < N - 1; ++j) a.m[j] = 0;
+        if (kind == 5) a.m[N - 1] = 0x80000000u;
     }
     a.m[N - 1] |= 0x80000000u;
     int kclr = 32 * N - p;
@@ -326,6 +335,7 @@ int main(int argc, char** argv)
     check_p<3>(80, cases); check_chain<3>(80, cases / 10);
     check_p<6>(176, cases); check_p<6>(184, cases); check_chain<6>(184, cases / 10);
     check_p<10>(320, cases); check_chain<10>(320, cases / 10);
+    printf("short-product fallbacks=%ld\n", mdzg::mdzg_short_fallbacks);
     printf("%s fails=%ld\n", fails ? "FAIL" : "OK", fails);
     return fails ? 1 : 0;
 }

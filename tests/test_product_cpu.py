@@ -101,3 +101,6 @@ def test_arithmetic_header_matches_mpfr(tmp_path):
                            timeout=300)
         assert r.returncode == 0, r.stderr[-2000:]
         assert "OK fails=0" in r.stdout
+        # the short products' exact fallback was exercised, not only the fast path
+        nfb = int(r.stdout.split("short-product fallbacks=")[1].split()[0])
+        assert nfb > 0
