@@ -324,7 +324,7 @@ MP_HD void mant_mul(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t&
 }
 
 /* Square with the off-diagonal products formed once and doubled per column
-   (n(n+1)/2 multiply-accumulates); pays off from N >= 24 words. */
+   (n(n+1)/2 multiply-accumulates); pays off from N >= 12 words. */
 template <int N, int K0>
 MP_HD void mant_sqr_sym(const uint32_t* a, uint32_t* H, uint32_t& S)
 {
@@ -391,7 +391,7 @@ MP_HD void mant_sqr_sym(const uint32_t* a, uint32_t* H, uint32_t& S)
    exact because the exact value is not a midpoint.  MPFR's own mulhigh
    path rests on the same bound-and-check argument. */
 #ifndef MDZG_SHORT_MIN
-#define MDZG_SHORT_MIN 6
+#define MDZG_SHORT_MIN 3
 #endif
 #if defined(MDZG_COUNT_SHORT_FALLBACK) && !defined(__HIP_DEVICE_COMPILE__)
 extern long mdzg_short_fallbacks;   /* host tests: how often the check failed */
@@ -408,7 +408,7 @@ constexpr int short_col()
 template <int N, bool SQR, int K0>
 MP_HD void mant_product(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t& S)
 {
-    if (SQR && N >= 24)
+    if (SQR && N >= 12)
         mant_sqr_sym<N, K0>(a, H, S);
     else
         mant_mul<N, K0>(a, SQR ? a : b, H, S);
