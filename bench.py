@@ -239,6 +239,18 @@ def config5_deep(ctx, M, WL, np, cpu=True):
     return out
 
 
+def rn1024_band(ctx, WL, rows):
+    """Rows 1048.. of the config 5 frame (3840 samples each, p=1024, depth 20000)."""
+    v5 = WL.config5_view()
+    ctx.render(v5, lines=(1048, 2), cols=(0, 3840))
+    _, st5 = ctx.render(v5, lines=(1048, rows), cols=(0, 3840))
+    return dict(workload=f"config5 rows 1048-{1047 + rows} x 3840 samples, p=1024, depth 20000",
+                pixel_iters=st5["pixel_iters"], kernel_ms=st5["kernel_ms"],
+                kernel_gpix_per_s=st5["pixel_iters"] / (st5["kernel_ms"] * 1e-3) / 1e9,
+                frac_peak=2 * 2080 * st5["pixel_iters"] / (st5["kernel_ms"] * 1e-3)
+                / 1e12 / PEAK_TOPS)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,6 +260,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the config 2 timed steps (for counter runs)")
+    ap.add_argument("--band-only", type=int, default=0, metavar="ROWS",
+                    help="only the 1024-bit figure over ROWS rows of config 5 (for counter runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only a plain device-to-device copy of MIB MiB (calibrates "
                          "FETCH_SIZE / WRITE_SIZE against known bytes)")
@@ -274,6 +288,11 @@ def main():
     from mdz_amd import workloads as WL
 
     ctx = M.Context(0)
+    if a.band_only:
+        r = rn1024_band(ctx, WL, a.band_only)
+        ctx.close()
+        print(json.dumps(r))
+        return
     res = dict(metric=METRIC, unit="G pixel-iter/s", n_gpus=1, higher_is_better=True,
                scaling="weak", vs_baseline=None, data="synthetic")
 
@@ -334,14 +353,7 @@ def main():
         log(f"fp64: {res['fp64']['kernel_gpix_per_s']:.1f} G pixel-iter/s (kernel)")
 
         # 1024-bit figure: a band of config 5 rows
-        v5 = WL.config5_view()
-        _, st5 = ctx.render(v5, lines=(1048, 2), cols=(0, 3840))
-        _, st5 = ctx.render(v5, lines=(1048, 64), cols=(0, 3840))
-        res["rn1024"] = dict(workload="config5 rows 1048-1111 x 3840 samples, p=1024, depth 20000",
-                             pixel_iters=st5["pixel_iters"], kernel_ms=st5["kernel_ms"],
-                             kernel_gpix_per_s=st5["pixel_iters"] / (st5["kernel_ms"] * 1e-3) / 1e9,
-                             frac_peak=2 * 2080 * st5["pixel_iters"] / (st5["kernel_ms"] * 1e-3)
-                             / 1e12 / PEAK_TOPS)
+        res["rn1024"] = rn1024_band(ctx, WL, 64)
         log(f"rn1024: {res['rn1024']['kernel_gpix_per_s']:.3f} G pixel-iter/s (kernel)")
 
         # p50/p99 tile latency: continuous batching, 64x64 config 2 crops

@@ -262,6 +262,23 @@ MP_HD void acc96_add(uint64_t& acc, uint32_t& c2, uint64_t accb, uint32_t c2b)
     acc = ((uint64_t)l1 << 32) | l0;
 }
 
+/* two multiply-accumulates in one asm block (one hazard gap instead of two) */
+MP_HD void mac96x2(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint64_t& acc, uint32_t& c2)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t cc;
+    asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
+        "v_addc_co_u32 %2, %1, 0, %2, %1\n\t"
+        "v_mad_u64_u32 %0, %1, %5, %6, %0\n\t"
+        "v_addc_co_u32 %2, %1, 0, %2, %1"
+        : "+v"(acc), "=&s"(cc), "+v"(c2)
+        : "v"(a0), "v"(b0), "v"(a1), "v"(b1));
+#else
+    mac96(a0, b0, acc, c2);
+    mac96(a1, b1, acc, c2);
+#endif
+}
+
 /* first product of a column: the third accumulator word starts from the carry */
 MP_HD void mac96_first(uint32_t a, uint32_t b, uint64_t& acc, uint32_t& c2)
 {
@@ -297,17 +314,28 @@ MP_HD void mant_mul(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t&
             xc[q] = 0;       /* set by the chain's first multiply-accumulate */
         }
         int n = 0;
+        int pend = -1;   /* K == 1: a product waiting for its pair */
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             const int j = k - i;
             if (j >= 0 && j < N) {
-                if (n < K)   /* first product of chain n */
+                if (n < K) {  /* first product of chain n */
                     mac96_first(a[i], b[j], xa[n % K], xc[n % K]);
-                else
+                } else if (K == 1) {
+                    if (pend < 0) {
+                        pend = i;
+                    } else {
+                        mac96x2(a[pend], b[k - pend], a[i], b[j], xa[0], xc[0]);
+                        pend = -1;
+                    }
+                } else {
                     mac96(a[i], b[j], xa[n % K], xc[n % K]);
+                }
                 ++n;
             }
         }
+        if (K == 1 && pend >= 0)
+            mac96(a[pend], b[k - pend], xa[0], xc[0]);
 #pragma unroll
         for (int q = 1; q < K; ++q)
             if (q < n) acc96_add(xa[0], xc[0], xa[q], xc[q]);
@@ -341,14 +369,28 @@ MP_HD void mant_sqr_sym(const uint32_t* a, uint32_t* H, uint32_t& S)
             xc[q] = 0;
         }
         int n = 0;
+        int pend = -1;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             const int j = k - i;
             if (j > i && j < N) {
-                mac96(a[i], a[j], xa[n % K], xc[n % K]);
+                if (n < K) {
+                    mac96_first(a[i], a[j], xa[n % K], xc[n % K]);
+                } else if (K == 1) {
+                    if (pend < 0) {
+                        pend = i;
+                    } else {
+                        mac96x2(a[pend], a[k - pend], a[i], a[j], xa[0], xc[0]);
+                        pend = -1;
+                    }
+                } else {
+                    mac96(a[i], a[j], xa[n % K], xc[n % K]);
+                }
                 ++n;
             }
         }
+        if (K == 1 && pend >= 0)
+            mac96(a[pend], a[k - pend], xa[0], xc[0]);
 #pragma unroll
         for (int q = 1; q < K; ++q)
             if (q < n) acc96_add(xa[0], xc[0], xa[q], xc[q]);
