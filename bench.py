@@ -353,7 +353,7 @@ def main():
         log(f"fp64: {res['fp64']['kernel_gpix_per_s']:.1f} G pixel-iter/s (kernel)")
 
         # 1024-bit figure: a band of config 5 rows
-        res["rn1024"] = rn1024_band(ctx, WL, 64)
+        res["rn1024"] = rn1024_band(ctx, WL, 192)
         log(f"rn1024: {res['rn1024']['kernel_gpix_per_s']:.3f} G pixel-iter/s (kernel)")
 
         # p50/p99 tile latency: continuous batching, 64x64 config 2 crops
