@@ -103,6 +103,14 @@ static void rand_op(mp<N>& a, int p, int emin, int emax)
     a.m[N - 1] |= 0x80000000u;
     int kclr = 32 * N - p;
     for (int b = 0; b < kclr; ++b) a.m[b >> 5] &= ~(1u << (b & 31));
+    if (kind == 6 && kclr % 2 == 0) {
+        // 2^(32N-1) + 2^(16N + kclr/2 - 1): its square is an exact tie at p
+        // with an even last bit
+        for (int j = 0; j < N; ++j) a.m[j] = 0;
+        const int t = 16 * N + kclr / 2 - 1;
+        a.m[N - 1] = 0x80000000u;
+        a.m[t >> 5] |= 1u << (t & 31);
+    }
     a.e = emin + (int)(rnd32() % (uint32_t)(emax - emin + 1));
     a.s = rnd32() & 1u;
     if (kind == 3) set_zero(a);

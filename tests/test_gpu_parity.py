@@ -77,6 +77,27 @@ def test_every_precision_class(ctx, oracle, p):
         assert _eq(got, ref), (p, fractal)
 
 
+@pytest.mark.parametrize("p", [96, 100, 128, 200, 256, 500, 512, 1000, 1024])
+def test_short_product_fallback(ctx, oracle, p):
+    """Start values that make the first products exact put their guard word on
+    the rounding boundary, where the short products must recompute in full:
+    for ragged p dyadic c (guard word 0); for p = 32N, c = 1/2 + 2^(-1-16N),
+    whose square is an exact tie (guard word 2^31) in column 0."""
+    from fractions import Fraction as F
+    N = p // 32
+    if p == 32 * N:
+        xmin, width, ymax, W, H = F(1, 2) + F(1, 2 ** (1 + 16 * N)), F(1, 16), F(1, 8), 16, 8
+    else:
+        xmin, width, ymax, W, H = F(-2), F(4), F(3, 2), 32, 24
+    rect = [cases.hexs(xmin), cases.hexs(xmin + width), cases.hexs(ymax), cases.hexs(width)]
+    for fractal in (0, 1):
+        v = M.make_view(rect, precision=p, depth=200, width=W, height=H, fractal=fractal)
+        got, _ = ctx.render(v)
+        ref = oracle.render(rect, precision=p, depth=200, width=W, height=H, fractal=fractal,
+                            threads=16)
+        assert _eq(got, ref), (p, fractal)
+
+
 @pytest.mark.parametrize("fractal", [0, 1, 2, 3])
 def test_julia_family(ctx, oracle, fractal):
     J = cases.JULIA

@@ -104,3 +104,14 @@ def test_arithmetic_header_matches_mpfr(tmp_path):
         # the short products' exact fallback was exercised, not only the fast path
         nfb = int(r.stdout.split("short-product fallbacks=")[1].split()[0])
         assert nfb > 0
+
+
+def test_device_op_check_library_loads():
+    """tests/gpu_arith/libarith_check.so (built by __graft_entry__.build) loads
+    and exports its entry point; calling it needs a GPU."""
+    import ctypes
+    from tests.gpu_arith import build as AB
+    if not os.path.exists(AB.LIB):
+        pytest.skip("not built")
+    L = ctypes.CDLL(AB.LIB)
+    assert hasattr(L, "arith_check")
