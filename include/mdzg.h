@@ -54,10 +54,12 @@ enum { MDZG_MANDELBROT = 0, MDZG_BURNING_SHIP = 1, MDZG_GENERALIZED_CELTIC = 2,
 enum { MDZG_FAMILY_MANDEL = 0, MDZG_FAMILY_JULIA = 1 };
 /* arithmetic, the choice image_info_set_multi_prec makes (src/image_info.c:226-249):
    MPFR = fractal_mpfr_calculate_line, LD = fractal_calculate_line (long double),
-   DOUBLE = the same loop in IEEE double (the FP64 path) */
-enum { MDZG_PATH_MPFR = 0, MDZG_PATH_LD = 1, MDZG_PATH_DOUBLE = 2 };
+   DOUBLE = the same loop in IEEE double (the FP64 path),
+   GMP = fractal_gmp_calculate_line (src/fractal.c:260-397, GMP's truncating mpf
+   floats; multi-precision without rounding) */
+enum { MDZG_PATH_MPFR = 0, MDZG_PATH_LD = 1, MDZG_PATH_DOUBLE = 2, MDZG_PATH_GMP = 3 };
 
-#define MDZG_MAX_PRECISION 1024   /* largest p of the MPFR path built for the device */
+#define MDZG_MAX_PRECISION 1024   /* largest p of the MPFR and GMP paths built for the device */
 
 /* The image_info fields the line workers read (src/image_info.h:63-122). */
 typedef struct mdzg_view {
@@ -70,10 +72,11 @@ typedef struct mdzg_view {
     long long depth;        /* image_info.depth (1 .. INT32_MAX)                  */
     const char* xmin;       /* image_info.xmin, .xmax, .ymax, .width (held at     */
     const char* xmax;       /*   max(p, 80) bits, as image_info_set_precision     */
-    const char* ymax;       /*   keeps them, src/image_info.c:252-280)            */
-    const char* width;
+    const char* ymax;       /*   keeps them, src/image_info.c:252-280).  GMP path: */
+    const char* width;      /*   the coords' own values (mdzg_rect_from_center_coords) */
     const char* julia_re;   /* image_info.u.julia.c_re / c_im (Julia family only) */
-    const char* julia_im;
+    const char* julia_im;   /*   GMP path: read as MDZ holds them, c_re at max(p,80)
+                                 bits, c_im at 80 bits (src/image_info.c:63-64, :271-272) */
 } mdzg_view;
 
 /* A rectangle of a view: rows [line0, line0+nlines) x columns [col0, col0+ncols),
@@ -109,6 +112,13 @@ void mdzg_destroy(mdzg_ctx* ctx);
 int mdzg_rect_from_center(const char* cx, const char* cy, const char* size,
                           long precision, int user_w, int user_h,
                           char* xmin, char* xmax, char* ymax, char* width, int cap);
+
+/* The same rect as coords holds it, at the coords precision (max(p, 80) raised
+   to whole GMP limbs, src/coords.c:204-231): what coords_get_rect_gmp converts
+   for the GMP path (src/coords.c:317-324, src/render.c:36-37). */
+int mdzg_rect_from_center_coords(const char* cx, const char* cy, const char* size,
+                                 long precision, int user_w, int user_h,
+                                 char* xmin, char* xmax, char* ymax, char* width, int cap);
 
 /* Synchronous render of one tile: what fractal_*_calculate_line computes for
    those rows and columns (src/fractal.c:29-257), in one persistent launch. */

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "libmdzg.so")
 
 MANDELBROT, BURNING_SHIP, GENERALIZED_CELTIC, VARIANT = 0, 1, 2, 3
 FAMILY_MANDEL, FAMILY_JULIA = 0, 1
-PATH_MPFR, PATH_LD, PATH_DOUBLE = 0, 1, 2
+PATH_MPFR, PATH_LD, PATH_DOUBLE, PATH_GMP = 0, 1, 2, 3
 BATCH_PERSISTENT, BATCH_PER_TILE = 0, 1
 SERVE_BATCHED, SERVE_PER_REQUEST = 0, 1
 MAX_PRECISION = 1024
@@ -106,6 +106,7 @@ def lib():
         L.mdzg_destroy.restype = None
         L.mdzg_rect_from_center.argtypes = [ctypes.c_char_p] * 3 + [
             ctypes.c_long, ctypes.c_int, ctypes.c_int] + [ctypes.c_char_p] * 4 + [ctypes.c_int]
+        L.mdzg_rect_from_center_coords.argtypes = L.mdzg_rect_from_center.argtypes
         L.mdzg_render_tile.argtypes = [vp, P(Tile), P(Stats)]
         L.mdzg_render_batch.argtypes = [vp, P(Tile), ctypes.c_int, ctypes.c_int, P(Stats)]
         L.mdzg_submit.argtypes = [vp, P(Tile), ctypes.c_int, P(vp)]
@@ -146,13 +147,16 @@ def _check(rc, what):
     return rc
 
 
-def rect_from_center(cx, cy, size, precision, user_w, user_h):
+def rect_from_center(cx, cy, size, precision, user_w, user_h, coords=False):
     """img->xmin, xmax, ymax, width (hex strings at max(p,80) bits) of a centre view,
-    as the command-line path computes them (src/coords.c:204-313)."""
+    as the command-line path computes them (src/coords.c:204-313); with
+    coords=True the same rect at the coords precision, the GMP path's input
+    (src/coords.c:317-324)."""
     cap = 4096
     bufs = [ctypes.create_string_buffer(cap) for _ in range(4)]
-    _check(lib().mdzg_rect_from_center(_b(cx), _b(cy), _b(size), precision, user_w, user_h,
-                                       *bufs, cap), "rect_from_center")
+    fn = lib().mdzg_rect_from_center_coords if coords else lib().mdzg_rect_from_center
+    _check(fn(_b(cx), _b(cy), _b(size), precision, user_w, user_h, *bufs, cap),
+           "rect_from_center")
     return tuple(b.value.decode() for b in bufs)
 
 

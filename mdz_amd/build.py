@@ -2,8 +2,9 @@
 
     python -m mdz_amd.build [-j JOBS]
 
-The kernels are compiled once per word count N (k_rn.hip with -DMDZG_N=N), all
-objects in parallel, then linked with the host runtime against MPFR/GMP.
+The kernels are compiled once per word count N (k_rn.hip with -DMDZG_N=N), plus
+the FP64 and GMP-path objects, all in parallel, then linked with the host
+runtime against MPFR/GMP.
 """
 import argparse
 import concurrent.futures as cf
@@ -50,6 +51,8 @@ def _run(cmd):
 def jobs():
     hipcc = os.path.join(ROCM, "bin", "hipcc")
     steps = []
+    steps.append(([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "k_gmp.hip"),
+                   "-o", os.path.join(BUILD, "k_gmp.o")], "k_gmp.o"))
     steps.append(([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "k_fp64.hip"),
                    "-o", os.path.join(BUILD, "k_fp64.o")], "k_fp64.o"))
     for n in sorted(NWORDS, reverse=True):   # largest (slowest) first

@@ -2,7 +2,8 @@
  * launch.hip -- kernel table and launch wrappers called by the C host runtime.
  *
  * Kernel id: 0 = FP64; 1 + 2*i + full for the RN kernels of word count
- * MDZG_NWORDS[i] (full = the tile precision is exactly 32N bits).
+ * MDZG_NWORDS[i] (full = the tile precision is exactly 32N bits);
+ * MDZG_KID_GMP0 + j for the GMP-path kernels of storage bucket j.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -21,6 +22,7 @@ void mdzg_kget_rn12(int, const void**, const void**, const void**);
 void mdzg_kget_rn16(int, const void**, const void**, const void**);
 void mdzg_kget_rn24(int, const void**, const void**, const void**);
 void mdzg_kget_rn32(int, const void**, const void**, const void**);
+void mdzg_kget_gmp(int bucket, const void**, const void**, const void**);
 }
 
 typedef void (*kget_fn)(int, const void**, const void**, const void**);
@@ -37,6 +39,8 @@ static void lookup(int kid, int form, const void** f)
     const void* sl = nullptr;
     if (kid == 0)
         mdzg_kget_fp64(&t, &p, &sl);
+    else if (kid >= MDZG_KID_GMP0)
+        mdzg_kget_gmp(kid - MDZG_KID_GMP0, &t, &p, &sl);
     else
         kget_rn[(kid - 1) >> 1]((kid - 1) & 1, &t, &p, &sl);
     *f = form == 2 ? sl : (form == 1 ? p : t);
@@ -44,7 +48,7 @@ static void lookup(int kid, int form, const void** f)
 
 extern "C" {
 
-int mdzg_dev_nkernels(void) { return 1 + 2 * MDZG_NCLASS_N; }
+int mdzg_dev_nkernels(void) { return MDZG_NKERNELS; }
 
 int mdzg_dev_occupancy(int kid, int persist)
 {

@@ -8,7 +8,8 @@
  *
  * Stored multi-word values (mp<N> of mp_rn.h) take MDZG_VS(N) 32-bit words:
  *   w[0..N-1] significand (least significant first), w[N] exponent (int32),
- *   w[N+1] sign, padded to a multiple of 4 words for 16-byte loads.
+ *   w[N+1] sign, padded to a multiple of 4 words for 16-byte loads.  Values of
+ *   the GMP path (gf<L> of mpf_gmp.h) take MDZG_GF_VS(L) words.
  */
 #ifndef MDZG_DEV_H
 #define MDZG_DEV_H
@@ -17,10 +18,20 @@
 
 #define MDZG_VS(N) ((((N) + 2) + 3) & ~3)
 
-/* arithmetic classes: FP64, or the RN kernels for N words (N in MDZG_NWORDS) */
+/* arithmetic classes: FP64, the RN kernels for N words (N in MDZG_NWORDS),
+   and the GMP-path kernels for values of up to MDZG_GMP_LIMBS 64-bit limbs */
 #define MDZG_NCLASS_N 10
 #define MDZG_NWORDS_INIT { 1, 2, 3, 4, 6, 8, 12, 16, 24, 32 }
 #define MDZG_CLASS_FP64 (-1)
+#define MDZG_NGMP 3
+#define MDZG_GMP_LIMBS_INIT { 6, 10, 18 }
+/* kernel ids: 0 FP64, 1 + 2i + full for RN class i, 1 + 2 MDZG_NCLASS_N + j for
+   GMP bucket j */
+#define MDZG_NKERNELS (1 + 2 * MDZG_NCLASS_N + MDZG_NGMP)
+#define MDZG_KID_GMP0 (1 + 2 * MDZG_NCLASS_N)
+/* GMP-path stored value of L limbs: 2L words (limbs from the most
+   significant, low word first), size, exponent (limbs), sign; padded to 4 */
+#define MDZG_GF_VS(L) (((2 * (L) + 3) + 3) & ~3)
 
 typedef struct mdzg_dtile {
     uint32_t* out;              /* counts: nrows x ncols, row-major                  */
@@ -36,8 +47,8 @@ typedef struct mdzg_dtile {
     int32_t fractal, family;
     int32_t kclr;               /* RN: 32N - p bits cleared by rounding              */
     int32_t depth;              /* max iterations (image_info.depth)                 */
-    int32_t vs;                 /* RN: words per stored value                        */
-    int32_t pad0;
+    int32_t vs;                 /* RN, GMP: words per stored value                   */
+    int32_t gprec;              /* GMP: PREC + 1, the limbs a value may hold         */
     /* resumable slices (serving): per-pixel orbit state between slices */
     uint32_t* state;            /* z, z^2 per pixel: RN 4 stored values, FP64 4 doubles */
     int32_t* wzs;               /* iterations done per pixel (0 = not started)       */

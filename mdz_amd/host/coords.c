@@ -26,6 +26,13 @@
 #include "mdzg_host.h"
 
 const int mdzg_nwords[MDZG_NCLASS_N] = MDZG_NWORDS_INIT;
+const int mdzg_gmp_limbs[MDZG_NGMP] = MDZG_GMP_LIMBS_INIT;
+
+/* GMP's limb precision of mpf_init2(x, p): __GMPF_BITS_TO_PREC (gmp.h) */
+static long gmp_prec_limbs(long p)
+{
+    return ((p > 53 ? p : 53) + 2 * GMP_NUMB_BITS - 1) / GMP_NUMB_BITS;
+}
 
 int mdzg_class_of(int path, long precision, int* nw, int* kclr)
 {
@@ -33,6 +40,18 @@ int mdzg_class_of(int path, long precision, int* nw, int* kclr)
         *nw = 0;
         *kclr = 0;
         return 0;
+    }
+    if (path == MDZG_PATH_GMP) {
+        if (precision < 2 || precision > MDZG_MAX_PRECISION)
+            return -1;
+        const int lr = (int)gmp_prec_limbs(precision) + 1;   /* limbs a value may hold */
+        for (int j = 0; j < MDZG_NGMP; ++j)
+            if (mdzg_gmp_limbs[j] >= lr) {
+                *nw = lr;
+                *kclr = 0;
+                return MDZG_KID_GMP0 + j;
+            }
+        return -1;
     }
     long p = (path == MDZG_PATH_LD) ? 64 : precision;
     if (p < 1 || p > MDZG_MAX_PRECISION)
@@ -56,7 +75,7 @@ int mdzg_view_parse(const mdzg_view* v, mdzg_pview* pv)
 {
     memset(pv, 0, sizeof *pv);
     if (!v || v->fractal < 0 || v->fractal > 3 || v->family < 0 || v->family > 1 ||
-        v->path < 0 || v->path > 2 || v->real_width < 1 || v->real_height < 1 ||
+        v->path < 0 || v->path > MDZG_PATH_GMP || v->real_width < 1 || v->real_height < 1 ||
         v->depth < 1 || v->depth > INT32_MAX || v->precision < 1)
         return MDZG_ERR_ARG;
     int nw, kclr;
@@ -70,8 +89,28 @@ int mdzg_view_parse(const mdzg_view* v, mdzg_pview* pv)
     pv->nw = nw;
     pv->kclr = kclr;
     pv->hp = v->precision < 80 ? 80 : v->precision;
-    mpfr_inits2(pv->hp, pv->xmin, pv->xmax, pv->ymax, pv->width, pv->jre, pv->jim,
-                (mpfr_ptr)0);
+    if (v->path == MDZG_PATH_DOUBLE)
+        pv->vs = 0;
+    else if (v->path == MDZG_PATH_GMP)
+        pv->vs = MDZG_GF_VS(mdzg_gmp_limbs[kid - MDZG_KID_GMP0]);
+    else
+        pv->vs = MDZG_VS(nw);
+    /* the GMP path reads coords' own rect at the coords precision
+       (src/coords.c:204-231, :317-324) and image_info's Julia constant, whose
+       c_im keeps the 80 bits of its initialisation (src/image_info.c:63-64,
+       :271-272 change the precision of c_re twice and never that of c_im) */
+    long cp = pv->hp;
+    if (v->path == MDZG_PATH_GMP) {
+        mpf_t test;
+        mpf_init2(test, (mp_bitcnt_t)cp);
+        if ((long)mpf_get_prec(test) > cp)
+            cp = (long)mpf_get_prec(test);
+        mpf_clear(test);
+    }
+    mpfr_inits2(cp, pv->xmin, pv->xmax, pv->ymax, pv->width, (mpfr_ptr)0);
+    mpfr_init2(pv->jre, pv->hp);
+    mpfr_init2(pv->jim, v->path == MDZG_PATH_GMP ? 80 : pv->hp);
+    pv->parsed = 1;
     int bad = set_str(pv->xmin, v->xmin) | set_str(pv->xmax, v->xmax) |
               set_str(pv->ymax, v->ymax) | set_str(pv->width, v->width);
     if (v->family == MDZG_FAMILY_JULIA)
@@ -89,9 +128,34 @@ int mdzg_view_parse(const mdzg_view* v, mdzg_pview* pv)
 
 void mdzg_view_clear(mdzg_pview* pv)
 {
-    if (pv->hp)
+    if (pv->parsed)
         mpfr_clears(pv->xmin, pv->xmax, pv->ymax, pv->width, pv->jre, pv->jim, (mpfr_ptr)0);
-    pv->hp = 0;
+    pv->parsed = 0;
+}
+
+/* GMP-path stored value: limbs from the most significant (low word first),
+   then size, exponent and sign (mpf_gmp.h gf<store>) */
+void mdzg_store_gf(uint32_t* w, int store, mpf_srcptr x)
+{
+    memset(w, 0, sizeof(uint32_t) * (size_t)MDZG_GF_VS(store));
+    const int n = x->_mp_size < 0 ? -x->_mp_size : x->_mp_size;
+    for (int i = 0; i < n && i < store; ++i) {
+        const mp_limb_t l = x->_mp_d[n - 1 - i];
+        w[2 * i] = (uint32_t)l;
+        w[2 * i + 1] = (uint32_t)(l >> 32);
+    }
+    w[2 * store] = (uint32_t)n;
+    w[2 * store + 1] = (uint32_t)(int32_t)(n ? x->_mp_exp : 0);
+    w[2 * store + 2] = x->_mp_size < 0 ? 1u : 0u;
+}
+
+/* src/coords.c:13-18 with src/my_mpfr_to_str.c: the decimal string "%.Re" of
+   the MPFR value (its own precision decides the digits), read by mpf_set_str */
+static void mpfr_to_mpf(mpfr_srcptr from, mpf_ptr to)
+{
+    char d0[4096 + 1];                     /* MAX_DP, src/my_mpfr_to_str.h:10 */
+    mpfr_snprintf(d0, 4096, "%.Re", from);
+    mpf_set_str(to, d0, 10);
 }
 
 /* ---- stored-value conversion ------------------------------------------ */
@@ -137,7 +201,7 @@ int mdzg_coords_fill(const mdzg_pview* pv, int line0, int nlines, int col0, int 
                      uint32_t* xs, uint32_t* ys, uint32_t* jc)
 {
     const int nw = pv->nw;
-    const int vs = MDZG_VS(nw);
+    const int vs = pv->vs;
     const int W = pv->v.real_width;
     if (pv->v.path == MDZG_PATH_MPFR) {
         const mpfr_prec_t p = pv->v.precision;
@@ -167,6 +231,56 @@ int mdzg_coords_fill(const mdzg_pview* pv, int line0, int nlines, int col0, int 
             mdzg_store_mpfr(jc + vs, nw, t1);
         }
         mpfr_clears(t1, x, y, img_rw, img_xmin, width, step, (mpfr_ptr)0);
+        return MDZG_OK;
+    }
+    if (pv->v.path == MDZG_PATH_GMP) {
+        /* src/fractal.c:298-328 and image_info.c:274-277: gxmin/gymax/gwidth
+           at max(p, 80), the chain at p, all in GMP's mpf arithmetic */
+        const int store = mdzg_gmp_limbs[pv->kid - MDZG_KID_GMP0];
+        const mp_bitcnt_t p = (mp_bitcnt_t)pv->v.precision;
+        mpf_t gxmin, gymax, gwidth, t1, x, y, img_rw, img_xmin, width;
+        mpf_init2(gxmin, (mp_bitcnt_t)pv->hp);
+        mpf_init2(gymax, (mp_bitcnt_t)pv->hp);
+        mpf_init2(gwidth, (mp_bitcnt_t)pv->hp);
+        mpfr_to_mpf(pv->xmin, gxmin);
+        mpfr_to_mpf(pv->ymax, gymax);
+        mpfr_to_mpf(pv->width, gwidth);
+        mpf_init2(t1, p);
+        mpf_init2(x, p);
+        mpf_init2(y, p);
+        mpf_init2(img_rw, p);
+        mpf_init2(img_xmin, p);
+        mpf_init2(width, p);
+        mpf_set_si(img_rw, W);
+        mpf_set(img_xmin, gxmin);
+        mpf_set(width, gwidth);
+        for (int k = 0; k < nlines; ++k) {
+            mpf_div(t1, width, img_rw);                           /* fractal.c:307 */
+            mpf_mul_ui(t1, t1, (unsigned long)(line0 + k));       /* :309 */
+            mpf_sub(y, gymax, t1);                                /* :310 */
+            mdzg_store_gf(ys + (size_t)k * vs, store, y);
+        }
+        for (int k = 0; k < ncols; ++k) {
+            mpf_ui_div(t1, (unsigned long)(col0 + k), img_rw);    /* :323 */
+            mpf_mul(x, t1, width);                                /* :325 */
+            mpf_add(x, x, img_xmin);                              /* :326 */
+            mdzg_store_gf(xs + (size_t)k * vs, store, x);
+        }
+        if (jc && pv->v.family == MDZG_FAMILY_JULIA) {
+            mpfr_to_mpf(pv->jre, t1);                             /* :341-342 */
+            mdzg_store_gf(jc, store, t1);
+            mpfr_to_mpf(pv->jim, t1);
+            mdzg_store_gf(jc + vs, store, t1);
+        }
+        mpf_clear(gxmin);
+        mpf_clear(gymax);
+        mpf_clear(gwidth);
+        mpf_clear(t1);
+        mpf_clear(x);
+        mpf_clear(y);
+        mpf_clear(img_rw);
+        mpf_clear(img_xmin);
+        mpf_clear(width);
         return MDZG_OK;
     }
     if (pv->v.path == MDZG_PATH_LD) {
@@ -209,9 +323,30 @@ static void put_hex(char* dst, int cap, mpfr_srcptr v)
     mpfr_snprintf(dst, (size_t)cap, "%Ra", v);
 }
 
+static int rect_from_center(const char* cx_s, const char* cy_s, const char* size_s,
+                            long precision, int user_w, int user_h, int at_coords,
+                            char* xmin_o, char* xmax_o, char* ymax_o, char* width_o, int cap);
+
 int mdzg_rect_from_center(const char* cx_s, const char* cy_s, const char* size_s,
                           long precision, int user_w, int user_h,
                           char* xmin_o, char* xmax_o, char* ymax_o, char* width_o, int cap)
+{
+    return rect_from_center(cx_s, cy_s, size_s, precision, user_w, user_h, 0,
+                            xmin_o, xmax_o, ymax_o, width_o, cap);
+}
+
+int mdzg_rect_from_center_coords(const char* cx_s, const char* cy_s, const char* size_s,
+                                 long precision, int user_w, int user_h,
+                                 char* xmin_o, char* xmax_o, char* ymax_o, char* width_o,
+                                 int cap)
+{
+    return rect_from_center(cx_s, cy_s, size_s, precision, user_w, user_h, 1,
+                            xmin_o, xmax_o, ymax_o, width_o, cap);
+}
+
+static int rect_from_center(const char* cx_s, const char* cy_s, const char* size_s,
+                            long precision, int user_w, int user_h, int at_coords,
+                            char* xmin_o, char* xmax_o, char* ymax_o, char* width_o, int cap)
 {
     if (!cx_s || !cy_s || !size_s || !xmin_o || !xmax_o || !ymax_o || !width_o ||
         cap < 8 || user_w < 1 || user_h < 1 || precision < 2)
@@ -236,7 +371,7 @@ int mdzg_rect_from_center(const char* cx_s, const char* cy_s, const char* size_s
     }
     mpfr_t xmin, xmax, ymin, ymax, width, height, tmp, o;
     mpfr_inits2(cp, xmin, xmax, ymin, ymax, width, height, tmp, (mpfr_ptr)0);
-    mpfr_init2(o, hp);
+    mpfr_init2(o, at_coords ? cp : hp);
     if (aspect > 1.0) {                                     /* coords.c:272-285 */
         mpfr_set(width, size, MPFR_RNDN);
         mpfr_div_d(height, width, aspect, MPFR_RNDN);
@@ -256,7 +391,8 @@ int mdzg_rect_from_center(const char* cx_s, const char* cy_s, const char* size_s
         mpfr_sub(xmin, cx, tmp, MPFR_RNDN);
         mpfr_add(xmax, xmin, width, MPFR_RNDN);
     }
-    /* coords_get_rect: copied into image_info at max(p, 80) (coords.c:305-313) */
+    /* coords_get_rect: copied into image_info at max(p, 80) (coords.c:305-313);
+       or, for the GMP path, as coords holds them (coords.c:317-324) */
     mpfr_set(o, xmin, MPFR_RNDN);  put_hex(xmin_o, cap, o);
     mpfr_set(o, xmax, MPFR_RNDN);  put_hex(xmax_o, cap, o);
     mpfr_set(o, ymax, MPFR_RNDN);  put_hex(ymax_o, cap, o);

@@ -27,15 +27,20 @@ int mdzg_dev_launch_slice(int kid, const mdzg_dtile* d_tiles, int ntiles, uint64
                           unsigned long long* d_counter, int grid, int budget, hipStream_t st);
 
 extern const int mdzg_nwords[MDZG_NCLASS_N];
+extern const int mdzg_gmp_limbs[MDZG_NGMP];
 
-/* A parsed view: image_info fields as MPFR values at max(p, 80) bits. */
+/* A parsed view: image_info fields as MPFR values at max(p, 80) bits; on the
+   GMP path the coordinates at the coords precision, the Julia c_re at
+   max(p, 80) and c_im at 80 bits, as the reference holds them. */
 typedef struct mdzg_pview {
     mdzg_view v;              /* caller's scalars (string pointers not kept) */
     long hp;                  /* max(p, 80) */
     mpfr_t xmin, xmax, ymax, width, jre, jim;
     int kid;                  /* kernel id */
-    int nw;                   /* words N (RN classes), 0 for FP64 */
+    int nw;                   /* words N (RN classes), 0 for FP64; GMP: PREC + 1 limbs */
     int kclr;                 /* 32N - p */
+    int vs;                   /* words per stored value, 0 for FP64 */
+    int parsed;
 } mdzg_pview;
 
 int  mdzg_view_parse(const mdzg_view* v, mdzg_pview* pv);
@@ -53,46 +58,25 @@ int mdzg_coords_fill(const mdzg_pview* pv, int line0, int nlines, int col0, int 
 void mdzg_coords_fp64(const mdzg_pview* pv, double* xmin, double* ymax, double* width,
                       double* jre, double* jim);
 
+/* mpf value (at most `limbs` limbs in use) -> GMP-path stored value of a
+   bucket of `store` limbs */
+void mdzg_store_gf(uint32_t* w, int store, mpf_srcptr x);
 /* mpfr (precision <= 32*nw) -> stored value */
 void mdzg_store_mpfr(uint32_t* w, int nw, mpfr_srcptr x);
 /* long double (x87, 64-bit significand) -> stored value with nw >= 2 */
 void mdzg_store_ld(uint32_t* w, int nw, long double x);
 
-/* device context */
+/* device context (runtime.c embeds it with its stream and buffer pools) */
 struct mdzg_ctx {
     int device;
     int ncu;
-    hipStream_t stream;         /* synchronous batches */
+    hipStream_t stream;         /* created with the context; jobs use the pool */
     hipStream_t ctl;            /* cancel-flag writes */
     int* d_cancel_zero;         /* a device int that stays 0 */
-    int occ[1 + 2 * MDZG_NCLASS_N];     /* resident blocks per CU (persistent grid) */
-    int chunk[1 + 2 * MDZG_NCLASS_N];   /* iterations between lane refills */
-    pthread_mutex_t lock;       /* serialises users of the shared arena */
-    void* d_arena;
-    size_t d_cap;
-    void* h_stage;
-    size_t h_cap;
-    hipEvent_t ev0, ev1;
+    int occ[MDZG_NKERNELS];     /* resident blocks per CU (persistent grid) */
+    int chunk[MDZG_NKERNELS];   /* iterations between lane refills */
+    pthread_mutex_t lock;
 };
-
-/* a staged batch on the device */
-typedef struct mdzg_plan {
-    int ntiles;
-    int nclass;                 /* classes present */
-    int cls_kid[1 + 2 * MDZG_NCLASS_N];
-    int cls_first[1 + 2 * MDZG_NCLASS_N];   /* index into the sorted tile order */
-    int cls_count[1 + 2 * MDZG_NCLASS_N];
-    uint64_t cls_total[1 + 2 * MDZG_NCLASS_N];
-    size_t off_desc, off_ctr, off_coords, off_out, bytes;
-    size_t out_bytes;
-    int* order;                 /* sorted tile order (by class) */
-    size_t* out_off;            /* per tile (original index): offset of its counts */
-    mdzg_dtile* desc;           /* host copy (in the staging buffer) */
-} mdzg_plan;
-
-int  mdzg_plan_build(mdzg_ctx* ctx, const mdzg_tile* tiles, int ntiles, mdzg_plan* pl,
-                     void** h_stage, size_t* h_cap, void* d_base_hint);
-void mdzg_plan_free(mdzg_plan* pl);
 
 double mdzg_now_ms(void);
 

@@ -58,10 +58,10 @@ struct mdzg_job {
     size_t off_ctr, off_coords, off_out, bytes, out_bytes;
     size_t* out_off;
     int nkid;
-    int kids[1 + 2 * MDZG_NCLASS_N];
+    int kids[MDZG_NKERNELS];
     hipEvent_t ev0, ev1;
     hipEvent_t ev_fork;
-    hipEvent_t ev_cls[1 + 2 * MDZG_NCLASS_N];
+    hipEvent_t ev_cls[MDZG_NKERNELS];
     double t_start;
     int launched;
     int done;
@@ -275,7 +275,7 @@ static void job_free_mem(mdzg_job* j)
     if (j->ev0) hipEventDestroy(j->ev0);
     if (j->ev1) hipEventDestroy(j->ev1);
     if (j->ev_fork) hipEventDestroy(j->ev_fork);
-    for (int k = 0; k < 1 + 2 * MDZG_NCLASS_N; ++k)
+    for (int k = 0; k < MDZG_NKERNELS; ++k)
         if (j->ev_cls[k]) hipEventDestroy(j->ev_cls[k]);
     if (j->slot >= 0) {
         release(j->ctx, j->slot);
@@ -349,9 +349,9 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
     off = ALIGN_UP(off + 8 * (size_t)(j->nkid + 1), 256);   /* counters + cancel flag */
     j->off_coords = off;
     for (int i = 0; i < ntiles; ++i) {
-        if (pv[i].nw) {
+        if (pv[i].vs) {
             size_t nval = (size_t)tiles[i].ncols + (size_t)tiles[i].nlines + 2;
-            off += ALIGN_UP(nval * 4 * (size_t)MDZG_VS(pv[i].nw), 16);
+            off += ALIGN_UP(nval * 4 * (size_t)pv[i].vs, 16);
         }
     }
     off = ALIGN_UP(off, 256);
@@ -405,11 +405,12 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
             d->family = t->view.family;
             d->kclr = pv[i].kclr;
             d->depth = (int32_t)t->view.depth;
-            d->vs = pv[i].nw ? MDZG_VS(pv[i].nw) : 0;
+            d->vs = pv[i].vs;
+            d->gprec = pv[i].v.path == MDZG_PATH_GMP ? pv[i].nw : 0;
             d->pix_begin = pix_begin;
             pix_begin += (uint64_t)t->ncols * (uint64_t)t->nlines;
-            if (pv[i].nw) {
-                const size_t vsb = 4 * (size_t)MDZG_VS(pv[i].nw);
+            if (pv[i].vs) {
+                const size_t vsb = 4 * (size_t)pv[i].vs;
                 uint32_t* xs = (uint32_t*)(H + coff);
                 uint32_t* ys = (uint32_t*)(H + coff + vsb * (size_t)t->ncols);
                 uint32_t* jc = (uint32_t*)(H + coff + vsb * ((size_t)t->ncols + (size_t)t->nlines));

@@ -37,7 +37,7 @@
 
 #define SENT 0xffffffffu
 #define ALIGN_UP(x, a) (((x) + (size_t)(a) - 1) & ~((size_t)(a) - 1))
-#define MAX_KID (1 + 2 * MDZG_NCLASS_N)
+#define MAX_KID MDZG_NKERNELS
 
 typedef struct sreq {
     uint64_t id;
@@ -168,8 +168,8 @@ static size_t req_bytes(const sreq* r)
 {
     const size_t npix = r->npix;
     size_t b = 0;
-    if (r->pv.nw) {
-        const size_t vsb = 4 * (size_t)MDZG_VS(r->pv.nw);
+    if (r->pv.vs) {
+        const size_t vsb = 4 * (size_t)r->pv.vs;
         b += ALIGN_UP(((size_t)r->tile.ncols + (size_t)r->tile.nlines + 2) * vsb, 256);
         b += ALIGN_UP(npix * 4 * vsb, 256);              /* state: 4 values */
     } else {
@@ -214,9 +214,9 @@ static int stage_new(mdzg_server* s, sreq** nw, int nnew, hipStream_t st)
     /* start values go through the pinned staging buffer */
     size_t need = 0;
     for (int i = 0; i < nnew; ++i)
-        if (nw[i]->pv.nw)
+        if (nw[i]->pv.vs)
             need += ALIGN_UP(((size_t)nw[i]->tile.ncols + (size_t)nw[i]->tile.nlines + 2) *
-                             4 * (size_t)MDZG_VS(nw[i]->pv.nw), 256);
+                             4 * (size_t)nw[i]->pv.vs, 256);
     if (need > s->capstage) {
         if (s->h_stage) {
             hipStreamSynchronize(st);
@@ -235,8 +235,8 @@ static int stage_new(mdzg_server* s, sreq** nw, int nnew, hipStream_t st)
         const mdzg_tile* t = &r->tile;
         char* base = s->d_arena + r->off;
         size_t o = 0;
-        if (r->pv.nw) {
-            const size_t vsb = 4 * (size_t)MDZG_VS(r->pv.nw);
+        if (r->pv.vs) {
+            const size_t vsb = 4 * (size_t)r->pv.vs;
             const size_t cb = ((size_t)t->ncols + (size_t)t->nlines + 2) * vsb;
             char* H = (char*)s->h_stage + hoff;
             uint32_t* xs = (uint32_t*)H;
@@ -319,7 +319,8 @@ static int run_slice(mdzg_server* s)
             d->family = r->tile.view.family;
             d->kclr = r->pv.kclr;
             d->depth = (int32_t)r->tile.view.depth;
-            d->vs = r->pv.nw ? MDZG_VS(r->pv.nw) : 0;
+            d->vs = r->pv.vs;
+            d->gprec = r->tile.view.path == MDZG_PATH_GMP ? r->pv.nw : 0;
             d->state = r->d_state;
             d->wzs = r->d_wzs;
             d->remaining = s->d_remaining + r->slot;
@@ -582,7 +583,9 @@ int mdzg_server_create(mdzg_ctx* ctx, const mdzg_server_cfg* cfg, mdzg_server** 
             const int lanes = ctx->ncu * s->occ_slice[k] * 256;
             s->budget_pix[k] = (long long)(s->cfg.occupancy_budget * (double)lanes);
             int nw = 0;
-            if (k > 0)
+            if (k >= MDZG_KID_GMP0)       /* GMP path: limbs of 64 bits */
+                nw = 2 * mdzg_gmp_limbs[k - MDZG_KID_GMP0];
+            else if (k > 0)
                 nw = mdzg_nwords[(k - 1) >> 1];
             s->slice_iters[k] = s->cfg.slice_iters > 0 ? s->cfg.slice_iters : default_slice_iters(nw);
         }

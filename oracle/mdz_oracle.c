@@ -41,6 +41,7 @@ typedef int64_t depth_t;
 /* ------------------------------------------------------------------ */
 typedef depth_t (*ld_fn)(depth_t, long double, long double, long double,
                          long double, long double, long double);
+typedef depth_t (*gmp_fn)(depth_t, mpf_t, mpf_t, mpf_t, mpf_t, mpf_t, mpf_t, mpf_t, mpf_t);
 typedef depth_t (*mp_fn)(depth_t, mpfr_t, mpfr_t, mpfr_t, mpfr_t, mpfr_t,
                          mpfr_t, mpfr_t, mpfr_t);
 
@@ -161,10 +162,40 @@ static depth_t mp_loop(int fractal, depth_t depth, mpfr_t bail,
     return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* GMP loops: src/frac_mandel.c:55-80, src/frac_burning_ship.c:58-83,     */
+/* src/frac_generalized_celtic.c:58-83, src/frac_variant.c:58-87.  The     */
+/* same operation order in GMP's truncating mpf arithmetic (mpf_mul,       */
+/* mpf_mul_ui, mpf_add, mpf_sub, mpf_abs, mpf_cmp of GMP 6.2.1).          */
+/* ------------------------------------------------------------------ */
+static depth_t gmp_loop(int fractal, depth_t depth, mpf_t bail,
+                        mpf_t wim, mpf_t wre, mpf_t c_im, mpf_t c_re,
+                        mpf_t wim2, mpf_t wre2, mpf_t t1)
+{
+    for (depth_t wz = 1; wz <= depth; wz++) {
+        mpf_mul(t1, wre, wim);
+        if (fractal == ORC_BURNING_SHIP)
+            mpf_abs(t1, t1);
+        mpf_mul_ui(t1, t1, 2);
+        mpf_add(wim, t1, c_im);
+        mpf_sub(t1, wre2, wim2);
+        if (fractal == ORC_GENERALIZED_CELTIC || (fractal == ORC_VARIANT && (wz % 2)))
+            mpf_abs(t1, t1);
+        mpf_add(wre, t1, c_re);
+        mpf_mul(wim2, wim, wim);
+        mpf_mul(wre2, wre, wre);
+        mpf_add(t1, wim2, wre2);
+        if (mpf_cmp(t1, bail) > 0)
+            return wz;
+    }
+    return 0;
+}
+
 /* compiled reference loops, when loaded (oracle/_ref/libfrac_ref.so) */
 static void*  ref_handle = NULL;
 static ld_fn  ref_ld[4];
 static mp_fn  ref_mp[4];
+static gmp_fn ref_gmp[4];
 
 int orc_version(void) { return 1; }
 
@@ -175,9 +206,12 @@ int orc_use_reference(const char* so_path)
     static const char* mp_names[4] = { "frac_mandel_mpfr", "frac_burning_ship_mpfr",
                                        "frac_generalized_celtic_mpfr",
                                        "frac_variant_mpfr" };
+    static const char* gmp_names[4] = { "frac_mandel_gmp", "frac_burning_ship_gmp",
+                                        "frac_generalized_celtic_gmp", "frac_variant_gmp" };
     if (!so_path) {
         memset(ref_ld, 0, sizeof ref_ld);
         memset(ref_mp, 0, sizeof ref_mp);
+        memset(ref_gmp, 0, sizeof ref_gmp);
         return 0;
     }
     if (!ref_handle)
@@ -187,9 +221,11 @@ int orc_use_reference(const char* so_path)
     for (int i = 0; i < 4; ++i) {
         ref_ld[i] = (ld_fn)dlsym(ref_handle, ld_names[i]);
         ref_mp[i] = (mp_fn)dlsym(ref_handle, mp_names[i]);
-        if (!ref_ld[i] || !ref_mp[i]) {
+        ref_gmp[i] = (gmp_fn)dlsym(ref_handle, gmp_names[i]);
+        if (!ref_ld[i] || !ref_mp[i] || !ref_gmp[i]) {
             memset(ref_ld, 0, sizeof ref_ld);
             memset(ref_mp, 0, sizeof ref_mp);
+            memset(ref_gmp, 0, sizeof ref_gmp);
             return -1;
         }
     }
@@ -214,6 +250,14 @@ static depth_t run_mp(int fractal, depth_t depth, mpfr_t bail, mpfr_t wim,
     return mp_loop(fractal, depth, bail, wim, wre, c_im, c_re, wim2, wre2, t1);
 }
 
+static depth_t run_gmp(int fractal, depth_t depth, mpf_t bail, mpf_t wim, mpf_t wre,
+                       mpf_t c_im, mpf_t c_re, mpf_t wim2, mpf_t wre2, mpf_t t1)
+{
+    if (ref_gmp[0])
+        return ref_gmp[fractal](depth, bail, wim, wre, c_im, c_re, wim2, wre2, t1);
+    return gmp_loop(fractal, depth, bail, wim, wre, c_im, c_re, wim2, wre2, t1);
+}
+
 /* ------------------------------------------------------------------ */
 /* Frame state shared by the workers                                    */
 /* ------------------------------------------------------------------ */
@@ -222,6 +266,7 @@ typedef struct job {
     int32_t* out;
     long hp;                 /* max(p, 80): precision of img->xmin etc. */
     mpfr_t xmin, xmax, ymax, width, jre, jim;
+    mpf_t gxmin, gymax, gwidth;   /* GMP path: img->gxmin/gymax/gwidth */
     pthread_mutex_t lock;    /* guards next_row (render_threads.c:366-369) */
     int next_row;
 } job;
@@ -268,6 +313,60 @@ static void line_mpfr(job* J, int line, int32_t* out)
     }
     mpfr_clears(x, y, x2, y2, c_re, c_im, wre, wim, wre2, wim2, bail, width,
                 img_rw, img_xmin, t1, (mpfr_ptr)0);
+}
+
+/* coords.c:13-18 (mpfr_to_gmp) with src/my_mpfr_to_str.c (MPFR >= 2.4):
+   the decimal string "%.Re" of the MPFR value, read by mpf_set_str */
+static void to_gmp(mpfr_t from, mpf_t to)
+{
+    char d0[4096 + 1];                  /* MAX_DP, src/my_mpfr_to_str.h:10 */
+    mpfr_snprintf(d0, 4096, "%.Re", from);
+    mpf_set_str(to, d0, 10);
+}
+
+/* fractal.c:260-397 for the selected columns of one row */
+static void line_gmp(job* J, int line, int32_t* out)
+{
+    const orc_frame* f = J->f;
+    mp_bitcnt_t p = (mp_bitcnt_t)f->precision;
+    mpf_t x, y, x2, y2, c_re, c_im, wre, wim, wre2, wim2, bail, width, img_rw, img_xmin, t1;
+    mpf_t* all[] = { &x, &y, &x2, &y2, &c_re, &c_im, &wre, &wim, &wre2, &wim2, &bail,
+                     &width, &img_rw, &img_xmin, &t1 };
+    for (size_t i = 0; i < sizeof all / sizeof all[0]; ++i)
+        mpf_init2(*all[i], p);
+
+    mpf_set_si(bail, 4);
+    mpf_set_si(img_rw, f->real_width);
+    mpf_set(img_xmin, J->gxmin);
+    mpf_set(width, J->gwidth);
+
+    mpf_div(t1, width, img_rw);                        /* fractal.c:307 */
+    mpf_mul_ui(t1, t1, (unsigned long)line);           /* :309 */
+    mpf_sub(y, J->gymax, t1);                          /* :310 */
+    mpf_mul(y2, y, y);                                 /* :311 */
+
+    for (int k = 0; k < f->ncols; ++k) {
+        int ix = f->col0 + k;
+        mpf_ui_div(t1, (unsigned long)ix, img_rw);     /* :323 */
+        mpf_mul(x, t1, width);                         /* :325 */
+        mpf_add(x, x, img_xmin);                       /* :326 */
+        mpf_mul(x2, x, x);                             /* :328 */
+        mpf_set(wre, x);
+        mpf_set(wim, y);
+        mpf_set(wre2, x2);
+        mpf_set(wim2, y2);
+        if (f->family == ORC_FAMILY_JULIA) {
+            to_gmp(J->jre, c_re);                      /* :341-342 */
+            to_gmp(J->jim, c_im);
+        } else {
+            mpf_set(c_re, x);
+            mpf_set(c_im, y);
+        }
+        out[k] = (int32_t)run_gmp(f->fractal, f->depth, bail, wim, wre, c_im, c_re,
+                                  wim2, wre2, t1);
+    }
+    for (size_t i = 0; i < sizeof all / sizeof all[0]; ++i)
+        mpf_clear(*all[i]);
 }
 
 /* fractal.c:29-117 for the selected columns of one row */
@@ -337,6 +436,8 @@ static void* worker(void* arg)
         int32_t* out = J->out + (size_t)k * f->ncols;
         if (f->path == ORC_PATH_MPFR)
             line_mpfr(J, line, out);
+        else if (f->path == ORC_PATH_GMP)
+            line_gmp(J, line, out);
         else if (f->path == ORC_PATH_LD)
             line_ld(J, line, out);
         else
@@ -360,7 +461,8 @@ int orc_render(const orc_frame* f, int nthreads, int32_t* out, double* elapsed_s
     if (!f || !out || f->fractal < 0 || f->fractal > 3 || f->real_width < 1 ||
         f->real_height < 1 || f->nlines < 0 || f->ncols < 0 || f->col0 < 0 ||
         f->col0 + f->ncols > f->real_width || f->depth < 1 ||
-        (f->path == ORC_PATH_MPFR && f->precision < MPFR_PREC_MIN))
+        f->path < 0 || f->path > ORC_PATH_GMP ||
+        ((f->path == ORC_PATH_MPFR || f->path == ORC_PATH_GMP) && f->precision < MPFR_PREC_MIN))
         return -1;
     if (nthreads < 1)
         nthreads = 1;
@@ -370,7 +472,23 @@ int orc_render(const orc_frame* f, int nthreads, int32_t* out, double* elapsed_s
     J.f = f;
     J.out = out;
     J.hp = f->precision < 80 ? 80 : f->precision;
-    mpfr_inits2(J.hp, J.xmin, J.xmax, J.ymax, J.width, J.jre, J.jim, (mpfr_ptr)0);
+    /* the GMP path reads the coordinates as coords holds them, at the coords
+       precision (src/coords.c:204-231, read by coords_get_rect_gmp :317-324),
+       and the Julia constant as image_info holds it: c_re at max(p, 80), c_im
+       at 80 bits (image_info.c:63-64 inits both at 80; :271-272 change the
+       precision of c_re twice and never that of c_im) */
+    const int gmp = f->path == ORC_PATH_GMP;
+    long cp = J.hp;
+    if (gmp) {
+        mpf_t test;
+        mpf_init2(test, (mp_bitcnt_t)cp);
+        if ((long)mpf_get_prec(test) > cp)
+            cp = (long)mpf_get_prec(test);
+        mpf_clear(test);
+    }
+    mpfr_inits2(gmp ? cp : J.hp, J.xmin, J.xmax, J.ymax, J.width, (mpfr_ptr)0);
+    mpfr_init2(J.jre, J.hp);
+    mpfr_init2(J.jim, gmp ? 80 : J.hp);
     int bad = set_str(J.xmin, f->xmin) | set_str(J.xmax, f->xmax) |
               set_str(J.ymax, f->ymax) | set_str(J.width, f->width);
     if (f->family == ORC_FAMILY_JULIA)
@@ -378,6 +496,15 @@ int orc_render(const orc_frame* f, int nthreads, int32_t* out, double* elapsed_s
     if (bad) {
         mpfr_clears(J.xmin, J.xmax, J.ymax, J.width, J.jre, J.jim, (mpfr_ptr)0);
         return -1;
+    }
+    if (gmp) {
+        /* image_info.c:274-277 (mpf at max(p, 80)), coords.c:317-324 */
+        mpf_init2(J.gxmin, (mp_bitcnt_t)J.hp);
+        mpf_init2(J.gymax, (mp_bitcnt_t)J.hp);
+        mpf_init2(J.gwidth, (mp_bitcnt_t)J.hp);
+        to_gmp(J.xmin, J.gxmin);
+        to_gmp(J.ymax, J.gymax);
+        to_gmp(J.width, J.gwidth);
     }
     pthread_mutex_init(&J.lock, NULL);
 
@@ -398,15 +525,42 @@ int orc_render(const orc_frame* f, int nthreads, int32_t* out, double* elapsed_s
 
     free(th);
     pthread_mutex_destroy(&J.lock);
+    if (gmp) {
+        mpf_clear(J.gxmin);
+        mpf_clear(J.gymax);
+        mpf_clear(J.gwidth);
+    }
     mpfr_clears(J.xmin, J.xmax, J.ymax, J.width, J.jre, J.jim, (mpfr_ptr)0);
     mpfr_free_cache();
     return 0;
 }
 
 /* coords.c:204-231 (precision), coords.c:265-313 (center_to_rect, get_rect) */
+static int rect_from_center(const char* cx_s, const char* cy_s, const char* size_s,
+                            long precision, int user_w, int user_h, int at_coords,
+                            char* xmin_o, char* xmax_o, char* ymax_o, char* width_o, int cap);
+
 int orc_rect_from_center(const char* cx_s, const char* cy_s, const char* size_s,
                          long precision, int user_w, int user_h,
                          char* xmin_o, char* xmax_o, char* ymax_o, char* width_o, int cap)
+{
+    return rect_from_center(cx_s, cy_s, size_s, precision, user_w, user_h, 0,
+                            xmin_o, xmax_o, ymax_o, width_o, cap);
+}
+
+/* the same rect as coords holds it, at the coords precision (GMP path input) */
+int orc_rect_from_center_coords(const char* cx_s, const char* cy_s, const char* size_s,
+                                long precision, int user_w, int user_h,
+                                char* xmin_o, char* xmax_o, char* ymax_o, char* width_o,
+                                int cap)
+{
+    return rect_from_center(cx_s, cy_s, size_s, precision, user_w, user_h, 1,
+                            xmin_o, xmax_o, ymax_o, width_o, cap);
+}
+
+static int rect_from_center(const char* cx_s, const char* cy_s, const char* size_s,
+                            long precision, int user_w, int user_h, int at_coords,
+                            char* xmin_o, char* xmax_o, char* ymax_o, char* width_o, int cap)
 {
     if (user_w < 1 || user_h < 1 || precision < 2)
         return -1;
@@ -427,7 +581,7 @@ int orc_rect_from_center(const char* cx_s, const char* cy_s, const char* size_s,
         return -1;
     }
     mpfr_inits2(cp, xmin, xmax, ymin, ymax, width, height, tmp, (mpfr_ptr)0);
-    mpfr_init2(o, hp);
+    mpfr_init2(o, at_coords ? cp : hp);
     if (aspect > 1.0) {
         mpfr_set(width, size, MPFR_RNDN);
         mpfr_div_d(height, width, aspect, MPFR_RNDN);

@@ -11,7 +11,10 @@
  *   formula loops, MPFR        : src/frac_mandel.c:25-52, src/frac_burning_ship.c:27-55,
  *                                src/frac_generalized_celtic.c:27-55, src/frac_variant.c:26-55
  *   formula loops, double      : the long double loops with `double` (the "CPU double loop")
- *   line workers               : src/fractal.c:29-117 (long double), src/fractal.c:120-257 (MPFR)
+ *   formula loops, GMP (mpf)   : src/frac_mandel.c:55-80, src/frac_burning_ship.c:58-83,
+ *                                src/frac_generalized_celtic.c:58-83, src/frac_variant.c:58-87
+ *   line workers               : src/fractal.c:29-117 (long double), src/fractal.c:120-257 (MPFR),
+ *                                src/fractal.c:260-397 (GMP), coords.c:13-18 (MPFR to mpf)
  *   rect from centre/size      : src/coords.c:204-231, :265-313, src/image_info.c:252-280
  *   line pool                  : src/render_threads.c:342-393 (dynamic line claim under a mutex)
  *
@@ -30,7 +33,7 @@ extern "C" {
 
 enum { ORC_MANDELBROT = 0, ORC_BURNING_SHIP = 1, ORC_GENERALIZED_CELTIC = 2, ORC_VARIANT = 3 };
 enum { ORC_FAMILY_MANDEL = 0, ORC_FAMILY_JULIA = 1 };
-enum { ORC_PATH_MPFR = 0, ORC_PATH_LD = 1, ORC_PATH_DOUBLE = 2 };
+enum { ORC_PATH_MPFR = 0, ORC_PATH_LD = 1, ORC_PATH_DOUBLE = 2, ORC_PATH_GMP = 3 };
 
 typedef struct orc_frame {
     int fractal;            /* ORC_MANDELBROT .. ORC_VARIANT                      */
@@ -41,7 +44,9 @@ typedef struct orc_frame {
     long precision;         /* image_info.precision (p of the MPFR loop)          */
     long long depth;        /* image_info.depth (max_iter)                        */
     /* image_info.xmin/xmax/ymax/width as MPFR-parsable strings (decimal or
-       hex-float); they are rounded to max(p, 80) bits as image_info holds them. */
+       hex-float); they are rounded to max(p, 80) bits as image_info holds them.
+       GMP path: the coords' own xmin/xmax/ymax/width, rounded to the coords
+       precision (src/coords.c:204-231), which coords_get_rect_gmp converts. */
     const char* xmin;
     const char* xmax;
     const char* ymax;
@@ -73,6 +78,10 @@ int  orc_render(const orc_frame* f, int nthreads, int32_t* out, double* elapsed_
 int  orc_rect_from_center(const char* cx, const char* cy, const char* size,
                           long precision, int user_w, int user_h,
                           char* xmin, char* xmax, char* ymax, char* width, int cap);
+/* the same rect at the coords precision (what coords_get_rect_gmp reads) */
+int  orc_rect_from_center_coords(const char* cx, const char* cy, const char* size,
+                                 long precision, int user_w, int user_h,
+                                 char* xmin, char* xmax, char* ymax, char* width, int cap);
 
 /* One formula loop on explicit start values (strings, rounded to p).
    Returns the count; writes the final wre/wim as hex strings if non-NULL. */

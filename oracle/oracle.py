@@ -14,7 +14,7 @@ REF_PATH = os.path.join(HERE, "_ref", "libfrac_ref.so")
 
 MANDELBROT, BURNING_SHIP, GENERALIZED_CELTIC, VARIANT = 0, 1, 2, 3
 FAMILY_MANDEL, FAMILY_JULIA = 0, 1
-PATH_MPFR, PATH_LD, PATH_DOUBLE = 0, 1, 2
+PATH_MPFR, PATH_LD, PATH_DOUBLE, PATH_GMP = 0, 1, 2, 3
 
 
 class Frame(ctypes.Structure):
@@ -53,6 +53,8 @@ def lib():
         L.orc_rect_from_center.argtypes = [ctypes.c_char_p] * 3 + [
             ctypes.c_long, ctypes.c_int, ctypes.c_int] + [ctypes.c_char_p] * 4 + [ctypes.c_int]
         L.orc_rect_from_center.restype = ctypes.c_int
+        L.orc_rect_from_center_coords.argtypes = L.orc_rect_from_center.argtypes
+        L.orc_rect_from_center_coords.restype = ctypes.c_int
         L.orc_formula_mpfr.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_long] + \
             [ctypes.c_char_p] * 6 + [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.orc_formula_mpfr.restype = ctypes.c_longlong
@@ -86,12 +88,14 @@ def _b(s):
     return None if s is None else (s.encode() if isinstance(s, str) else s)
 
 
-def rect_from_center(cx, cy, size, precision, user_w, user_h):
-    """(xmin, xmax, ymax, width) hex strings at max(p, 80) bits."""
+def rect_from_center(cx, cy, size, precision, user_w, user_h, coords=False):
+    """(xmin, xmax, ymax, width) hex strings at max(p, 80) bits (as image_info
+    holds them), or with coords=True at the coords precision (as coords holds
+    them: the input of the GMP path)."""
     cap = 4096
     bufs = [ctypes.create_string_buffer(cap) for _ in range(4)]
-    rc = lib().orc_rect_from_center(_b(cx), _b(cy), _b(size), precision, user_w, user_h,
-                                    *bufs, cap)
+    fn = lib().orc_rect_from_center_coords if coords else lib().orc_rect_from_center
+    rc = fn(_b(cx), _b(cy), _b(size), precision, user_w, user_h, *bufs, cap)
     if rc != 0:
         raise ValueError("bad rect arguments")
     return tuple(b.value.decode() for b in bufs)

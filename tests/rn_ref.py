@@ -104,8 +104,9 @@ def pixel_c(xmin, ymax, width, p, W, line, ix):
     return x, y
 
 
-def rect_from_center(cx, cy, size, p, user_w, user_h):
-    """src/coords.c:204-231 + :265-313 + coords_get_rect, exact RN arithmetic."""
+def rect_from_center(cx, cy, size, p, user_w, user_h, coords=False):
+    """src/coords.c:204-231 + :265-313 + coords_get_rect, exact RN arithmetic;
+    coords=True: the values at the coords precision (coords_get_rect_gmp)."""
     cp = max(p, 80)
     cp = max(cp, gmp_mpf_bits(cp))
     hp = max(p, 80)
@@ -129,4 +130,6 @@ def rect_from_center(cx, cy, size, p, user_w, user_h):
         tmp = rn(tmp * aspect, cp)
         xmin = rn(cx - tmp, cp)
         xmax = rn(xmin + width, cp)
+    if coords:
+        return (xmin, xmax, ymax, width)
     return tuple(rn(v, hp) for v in (xmin, xmax, ymax, width))

@@ -98,6 +98,65 @@ def test_short_product_fallback(ctx, oracle, p):
         assert _eq(got, ref), (p, fractal)
 
 
+# the GMP path: MDZ's truncating mpf loops (src/frac_mandel.c:55-80 and
+# siblings, src/fractal.c:260-397); these scenes include ones where its counts
+# differ from the MPFR path's (test, subdudehue, spiralblues)
+@pytest.mark.parametrize("name", ["test", "subdudehue", "spiralblues", "floral", "space_pad",
+                                  "honeytrace", "confirm"])
+def test_gmp_path_scenes(ctx, oracle, name):
+    rect, p, depth, fractal, _, (W, H) = cases.scene_rect(name)
+    depth = min(depth, 3000)
+    lines, cols = (H // 3, min(6, H // 2)), (0, W)
+    v = M.make_view(rect, precision=p, depth=depth, width=W, height=H, fractal=fractal,
+                    path=M.PATH_GMP)
+    got, _ = ctx.render(v, lines=lines, cols=cols)
+    ref = oracle.render(rect, precision=p, depth=depth, width=W, height=H, fractal=fractal,
+                        path=oracle.PATH_GMP, lines=(lines[0], lines[1], 1), cols=cols,
+                        threads=16)
+    assert _eq(got, ref), name
+
+
+@pytest.mark.parametrize("p", [2, 53, 64, 80, 128, 129, 192, 256, 257, 320, 512, 513, 640,
+                               1000, 1024])
+def test_gmp_path_every_precision(ctx, oracle, p):
+    """Every limb precision (PREC + 1 = 3 .. 18 limbs) in all three storage
+    buckets, centre-form rect at the coords precision."""
+    rect = oracle.rect_from_center("-1.7495931476527795538581440", "5.79069164258e-8",
+                                   "3e-9", p, 96, 72, coords=True)
+    for fractal in (0, 3):
+        v = M.make_view(rect, precision=p, depth=400, width=96, height=72, fractal=fractal,
+                        path=M.PATH_GMP)
+        got, _ = ctx.render(v, lines=(30, 4), cols=(0, 96))
+        ref = oracle.render(rect, precision=p, depth=400, width=96, height=72, fractal=fractal,
+                            path=oracle.PATH_GMP, lines=(30, 4, 1), threads=16)
+        assert _eq(got, ref), (p, fractal)
+
+
+def test_gmp_path_julia_and_batch(ctx, oracle):
+    """Julia on the GMP path, and GMP tiles of all buckets mixed with the other
+    arithmetics in one persistent batch and one launch per tile."""
+    J = cases.JULIA
+    tiles, refs = [], []
+    for p, fr in ((80, 0), (200, 1), (600, 2), (1024, 3)):
+        v = M.make_view(J["rect"], precision=p, depth=600, width=96, height=72, fractal=fr,
+                        family=M.FAMILY_JULIA, path=M.PATH_GMP, julia=J["c"])
+        tiles.append((v, (20, 6), (0, 96)))
+        refs.append(oracle.render(J["rect"], precision=p, depth=600, width=96, height=72,
+                                  fractal=fr, family=oracle.FAMILY_JULIA, path=oracle.PATH_GMP,
+                                  julia=J["c"], lines=(20, 6, 1), threads=16))
+    rect, p, depth, fractal, path, (W, H) = cases.scene_rect("subdudehue")
+    for pth in (M.PATH_MPFR, M.PATH_GMP):
+        v = M.make_view(rect, precision=p, depth=2000, width=W, height=H, fractal=fractal,
+                        path=pth)
+        tiles.append((v, (50, 5), (0, W)))
+        refs.append(oracle.render(rect, precision=p, depth=2000, width=W, height=H,
+                                  fractal=fractal, path=pth, lines=(50, 5, 1), threads=16))
+    outs_p, _ = ctx.render_batch(tiles, M.BATCH_PERSISTENT)
+    outs_t, _ = ctx.render_batch(tiles, M.BATCH_PER_TILE)
+    for a, b, r in zip(outs_p, outs_t, refs):
+        assert _eq(a, r) and _eq(b, r)
+
+
 @pytest.mark.parametrize("fractal", [0, 1, 2, 3])
 def test_julia_family(ctx, oracle, fractal):
     J = cases.JULIA

@@ -17,7 +17,7 @@ def _eq(a, b):
 
 def _mix():
     """Requests across every arithmetic: MPFR at aligned and ragged p, long
-    double, FP64, all four formulas, Julia."""
+    double, FP64, GMP, all four formulas, Julia."""
     reqs = []
     for name in ("test", "subdudehue", "satellite", "honeytrace", "spiralblues", "confirm",
                  "mdz_banner", "floral"):
@@ -27,6 +27,11 @@ def _mix():
         reqs.append((v, (H // 3, min(12, H // 3)), (W // 5, 48)))
     v1 = WL.config1_view(fractal=2)
     reqs.append((v1, (30, 40), (0, 240)))
+    for name in ("test", "floral"):              # the GMP path (truncating mpf)
+        rect, p, depth, fractal, _, (W, H) = cases.scene_rect(name)
+        v = M.make_view(rect, precision=p, depth=min(depth, 2000), width=W, height=H,
+                        fractal=fractal, path=M.PATH_GMP)
+        reqs.append((v, (H // 3, 6), (W // 5, 40)))
     J = cases.JULIA
     for fr in range(4):
         vj = M.make_view(J["rect"], precision=96, depth=800, width=96, height=72, fractal=fr,

@@ -175,16 +175,18 @@ def test_oracle_matches_golden(oracle, golden):
 
 @pytest.mark.skipif(REF_MISSING, reason="oracle/_ref is built only where the reference tree exists")
 def test_restated_loops_equal_reference_loops(oracle):
-    """Restated ld/MPFR loops == compiled src/frac_*.c on full rows of several views."""
+    """Restated ld/MPFR/GMP loops == compiled src/frac_*.c on full rows of several views."""
     import mdz_amd.workloads as WL
     v = WL.config1_view()
     r1 = [s.decode() for s in v._keep[:4]]
     jobs = [(r1, 64, 1000, 240, 180, 1, fr, 0, (None, None)) for fr in range(4)]
     for name in ("subdudehue", "test", "floral"):
         rect, p, depth, _, _, (W, H) = cases.scene_rect(name)
-        jobs += [(rect, p, min(depth, 3000), W, H, 0, fr, 0, (None, None)) for fr in range(4)]
+        jobs += [(rect, p, min(depth, 3000), W, H, path, fr, 0, (None, None))
+                 for fr in range(4) for path in (0, 3)]
     J = cases.JULIA
-    jobs += [(J["rect"], J["precision"], 500, 96, 72, 0, fr, 1, J["c"]) for fr in range(4)]
+    jobs += [(J["rect"], p, 500, 96, 72, path, fr, 1, J["c"]) for fr in range(4)
+             for path, p in ((0, J["precision"]), (3, J["precision"]), (3, 300))]
     for rect, p, depth, W, H, path, fr, fam, jc in jobs:
         kw = dict(precision=p, depth=depth, width=W, height=H, path=path, fractal=fr,
                   family=fam, julia=jc, lines=(H // 3, 3, 7), threads=8)
