@@ -1,6 +1,6 @@
 """Benchmark: G pixel-iterations/s of the escape-time path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2|1|3|4|5]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2|3|4|5|gmp]
                     [--no-cpu-baseline]
 
 A step is one pass of the hot path over one batch of synthetic input.  The
@@ -9,7 +9,8 @@ Mandelbrot on the MPFR path at p = 128, 640x480 at the seahorse-valley centre,
 width 1e-25, depth 5000; every pixel is inside, so a step is exactly
 640*480*5000 = 1.536e9 pixel-iterations.  The line also carries the FP64
 figure (config 1, 240x180, depth 1000), the 1024-bit figure (a band of
-config 5 rows) and, with --config 3/4, the batched and serving forms.
+config 5 rows) and, with --config 3/4/5/gmp, the batched and serving forms,
+the full deep render and the GMP path.
 
 pixel-iterations = sum over samples of (count if count > 0 else depth): the
 iterations the reference loop executes.  Algorithmic operations per
@@ -61,8 +62,8 @@ def host_cores():
     return min(n, cap) if cap > 0 else n
 
 
-def cpu_baseline(rect, precision, depth, W, H, line_step, threads):
-    """MDZ's MPFR loop on the host cores, timed on a sample of rows."""
+def cpu_baseline(rect, precision, depth, W, H, line_step, threads, path=0):
+    """MDZ's MPFR (or GMP) loop on the host cores, timed on a sample of rows."""
     from oracle import oracle as O
     kind = "reference"
     try:
@@ -72,7 +73,7 @@ def cpu_baseline(rect, precision, depth, W, H, line_step, threads):
         O.use_reference(False)
     nl = (H + line_step - 1) // line_step
     counts, el = O.render(rect, precision=precision, depth=depth, width=W, height=H,
-                          lines=(0, nl, line_step), threads=threads, timed=True)
+                          lines=(0, nl, line_step), threads=threads, timed=True, path=path)
     O.use_reference(False)
     pi = float(((counts > 0) * counts + (counts == 0) * depth).sum())
     return dict(value=pi / el / 1e9, unit="G pixel-iter/s", cores=threads, kind=kind,
@@ -108,6 +109,35 @@ def config3_batched(ctx, M, WL, np, ntiles=256):
                      persistent_kernel_ms=st_p["kernel_ms"], per_tile_kernel_ms=st_t["kernel_ms"],
                      launches_persistent=st_p["launches"], launches_per_tile=st_t["launches"],
                      identical_counts=bool(same)))
+
+
+def gmp_path(ctx, M, WL, a, cpu=True):
+    """MDZ's GMP path (truncating mpf, src/frac_mandel.c:55-80) on the config 2
+    view: full frames at p = 128 (4-limb values)."""
+    rect = M.rect_from_center(WL.CONFIG2_CENTRE[0], WL.CONFIG2_CENTRE[1], "1e-25", 128, 640,
+                              480, coords=True)
+    v = M.make_view(rect, precision=128, depth=5000, width=640, height=480, path=M.PATH_GMP)
+    ctx.render(v, lines=(0, 8), cols=(0, 640))
+    steps = max(1, min(a.steps, 3))
+    t0 = time.perf_counter()
+    kms, pis = [], []
+    for _ in range(steps):
+        _, st = ctx.render(v)
+        kms.append(st["kernel_ms"])
+        pis.append(st["pixel_iters"])
+    el = time.perf_counter() - t0
+    r = dict(value=sum(pis) / el / 1e9, steps=steps, warmup=1, ms_per_step=1e3 * el / steps,
+             dtype="u64 limbs, GMP mpf PREC 3 (p=128)",
+             config=dict(workload="config2 view on the GMP path (truncating mpf, p=128) "
+                         "640x480 depth 5000", kernel=f"k_gmp_persist<6>"),
+             kernel_gpix_per_s=pis[0] / (sum(kms) / len(kms) * 1e-3) / 1e9)
+    log(f"gmp path: {r['kernel_gpix_per_s']:.2f} G pixel-iter/s (kernel)")
+    if cpu:
+        r["cpu_baseline"] = cpu_baseline(list(rect), 128, 5000, 640, 480, 16, host_cores(),
+                                         path=3)
+        r["cpu_baseline"]["sample"] = r["cpu_baseline"]["sample"].replace("config 2", "config 2 GMP")
+        log(f"gmp cpu baseline: {r['cpu_baseline']['value']:.4f} G pixel-iter/s")
+    return r
 
 
 def crop_tiles(WL, n, seed=0x5E):
@@ -378,6 +408,8 @@ def main():
             log(f"cpu baseline: {res['cpu_baseline']['value']:.4f} G pixel-iter/s")
     elif a.config == "3":
         res.update(config3_batched(ctx, M, WL, np))
+    elif a.config == "gmp":
+        res.update(gmp_path(ctx, M, WL, a, cpu=not a.no_cpu_baseline))
     elif a.config == "5":
         r = config5_deep(ctx, M, WL, np, cpu=not a.no_cpu_baseline)
         res.update(value=r["gpix_per_s"], steps=1, warmup=0, ms_per_step=1e3 * r["seconds"],

@@ -240,11 +240,16 @@ extern "C" void mdzg_kget_gmp(int bucket, const void** tile, const void** persis
 {
     switch (bucket) {
     case 0:
+        *tile = (const void*)k_gmp_tile<4>;
+        *persist = (const void*)k_gmp_persist<4>;
+        *slice = (const void*)k_gmp_slice<4>;
+        break;
+    case 1:
         *tile = (const void*)k_gmp_tile<6>;
         *persist = (const void*)k_gmp_persist<6>;
         *slice = (const void*)k_gmp_slice<6>;
         break;
-    case 1:
+    case 2:
         *tile = (const void*)k_gmp_tile<10>;
         *persist = (const void*)k_gmp_persist<10>;
         *slice = (const void*)k_gmp_slice<10>;

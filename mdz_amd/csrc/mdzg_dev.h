@@ -23,8 +23,8 @@
 #define MDZG_NCLASS_N 10
 #define MDZG_NWORDS_INIT { 1, 2, 3, 4, 6, 8, 12, 16, 24, 32 }
 #define MDZG_CLASS_FP64 (-1)
-#define MDZG_NGMP 3
-#define MDZG_GMP_LIMBS_INIT { 6, 10, 18 }
+#define MDZG_NGMP 4
+#define MDZG_GMP_LIMBS_INIT { 4, 6, 10, 18 }
 /* kernel ids: 0 FP64, 1 + 2i + full for RN class i, 1 + 2 MDZG_NCLASS_N + j for
    GMP bucket j */
 #define MDZG_NKERNELS (1 + 2 * MDZG_NCLASS_N + MDZG_NGMP)

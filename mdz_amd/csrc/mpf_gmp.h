@@ -79,18 +79,70 @@ GF_HD void mul64(limb a, limb b, limb& hi, limb& lo)
 template <int L>
 GF_HD void gf_zero(gf<L>& r)
 {
+#pragma unroll
     for (int i = 0; i < L; ++i) r.t[i] = 0;
     r.size = 0;
     r.exp = 0;
     r.neg = 0;
 }
 
-/* limb i (from the top) of a window starting `off` limbs below the top of t,
-   holding n limbs; 0 outside */
-template <int L>
-GF_HD limb wl(const limb* t, int off, int n, int i)
+/* true when the predicate holds on some lane of the wave (device) / on
+   this value (host): products that are zero on every lane are skipped */
+GF_HD bool gf_any(bool c)
 {
-    return (i >= 0 && i < n && off + i < L) ? t[off + i] : (limb)0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ballot_w64(c) != 0;
+#else
+    return c;
+#endif
+}
+
+/* ---- fixed-index array helpers (the device keeps the arrays in registers:
+   every index below is a compile-time constant, run-time quantities only
+   select values) --------------------------------------------------------- */
+
+/* a[i] = 0 for i >= n */
+template <int L>
+GF_HD void keep_top(limb (&a)[L], int n)
+{
+#pragma unroll
+    for (int i = 0; i < L; ++i) a[i] = (i < n) ? a[i] : (limb)0;
+}
+
+/* shift toward the less significant end by k limbs (k >= 0), zeros enter at
+   the top and limbs pushed past position L-1 are lost */
+template <int L>
+GF_HD void shift_down(limb (&a)[L], int k)
+{
+#pragma unroll
+    for (int st = 1; st < 2 * L; st <<= 1) {
+        const bool c = (k & st) != 0;
+#pragma unroll
+        for (int i = L - 1; i >= 0; --i) a[i] = c ? ((i >= st) ? a[i - st] : (limb)0) : a[i];
+    }
+    if (k >= 2 * L) {
+#pragma unroll
+        for (int i = 0; i < L; ++i) a[i] = 0;
+    }
+}
+
+/* drop the top limb: a[i] = a[i+1] */
+template <int L>
+GF_HD void drop_top(limb (&a)[L])
+{
+#pragma unroll
+    for (int i = 0; i < L - 1; ++i) a[i] = a[i + 1];
+    a[L - 1] = 0;
+}
+
+/* a[i] for a run-time i in [0, L) (0 otherwise) */
+template <int L>
+GF_HD limb pick(const limb (&a)[L], int i)
+{
+    limb x = 0;
+#pragma unroll
+    for (int k = 0; k < L; ++k) x = (k == i) ? a[k] : x;
+    return x;
 }
 
 /* ------------------------------------------------------------------ */
@@ -103,34 +155,47 @@ GF_HD void gf_mul(gf<L>& r, const gf<L>& u, const gf<L>& v, int lr)
     const int P = lr - 1;
     const int us = u.size < P ? u.size : P;
     const int vs = v.size < P ? v.size : P;
-    if (us == 0 || vs == 0) {
-        gf_zero(r);
-        return;
+    limb U[L], V[L];
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        U[i] = (i < us) ? u.t[i] : (limb)0;
+        V[i] = (i < vs) ? v.t[i] : (limb)0;
     }
-    /* least significant first: U[i] = u.t[us-1-i] */
-    limb W[2 * L];
-    for (int k = 0; k < 2 * L; ++k) W[k] = 0;
-    for (int i = 0; i < us; ++i) {
-        const limb ui = u.t[us - 1 - i];
-        limb carry = 0;
-        for (int j = 0; j < vs; ++j) {
-            limb hi, lo, c1, c2;
-            mul64(ui, v.t[vs - 1 - j], hi, lo);
-            lo = addc(lo, W[i + j], 0, c1);
-            lo = addc(lo, carry, 0, c2);
-            W[i + j] = lo;
-            carry = hi + c1 + c2;
+    /* product of U and V as L-limb numbers, column by column from the least
+       significant limb; Z is the 2L-limb product, most significant first */
+    limb Z[2 * L];
+    limb a0 = 0, a1 = 0, a2 = 0;       /* 192-bit column accumulator */
+#pragma unroll
+    for (int q = 0; q < 2 * L - 1; ++q) {     /* q: LS-first column index */
+#pragma unroll
+        for (int i = 0; i < L; ++i) {
+            const int j = q - i;
+            /* U[L-1-i] and V[L-1-j] are zero unless L-1-i < us and L-1-j < vs;
+               in the small buckets, which often hold fewer limbs than they
+               store, a wave skips the products that are zero on all lanes */
+            if (j >= 0 && j < L && (L > 6 || gf_any((L - 1 - i < us) & (L - 1 - j < vs)))) {
+                limb hi, lo, c;
+                mul64(U[L - 1 - i], V[L - 1 - j], hi, lo);
+                a0 = addc(a0, lo, 0, c);
+                a1 = addc(a1, hi, c, c);
+                a2 += c;
+            }
         }
-        W[i + vs] = carry;
+        Z[2 * L - 1 - q] = a0;
+        a0 = a1;
+        a1 = a2;
+        a2 = 0;
     }
-    const int n = us + vs;
-    const int adj = (W[n - 1] == 0) ? 1 : 0;
-    const int rsize = n - adj;
-    const int keep = rsize < lr ? rsize : lr;
-    for (int k = 0; k < L; ++k) r.t[k] = (k < keep) ? W[n - 1 - adj - k] : 0;
+    Z[0] = a0;
+    const bool zero = (us == 0) | (vs == 0);
+    const int adj = (Z[0] == 0) ? 1 : 0;
+    const int rsize = us + vs - adj;
+    const int keep = zero ? 0 : (rsize < lr ? rsize : lr);
+#pragma unroll
+    for (int k = 0; k < L; ++k) r.t[k] = (k < keep) ? (adj ? Z[k + 1] : Z[k]) : (limb)0;
     r.size = keep;
-    r.exp = u.exp + v.exp - adj;
-    r.neg = u.neg ^ v.neg;
+    r.exp = zero ? 0 : u.exp + v.exp - adj;
+    r.neg = zero ? 0 : (u.neg ^ v.neg);
 }
 
 /* ------------------------------------------------------------------ */
@@ -143,36 +208,26 @@ GF_HD void gf_mul2(gf<L>& r, const gf<L>& u, int lr)
 {
     const int P = lr - 1;
     int size = u.size;
-    if (size == 0) {
-        gf_zero(r);
-        return;
-    }
     limb cin = 0;
     if (size > P) {
-        cin = u.t[P] >> 63;
+        cin = pick<L>(u.t, P) >> 63;
         size = P;
     }
     limb tmp[L];
     limb c = cin;
+#pragma unroll
     for (int i = L - 1; i >= 0; --i) {
-        if (i < size) {
-            tmp[i] = (u.t[i] << 1) | c;
-            c = u.t[i] >> 63;
-        } else {
-            tmp[i] = 0;
-        }
+        const bool in = i < size;
+        tmp[i] = in ? ((u.t[i] << 1) | c) : (limb)0;
+        c = in ? (u.t[i] >> 63) : c;
     }
-    r.neg = u.neg;
-    if (c) {
-        r.t[0] = 1;
-        for (int i = 1; i < L; ++i) r.t[i] = (i - 1 < size) ? tmp[i - 1] : 0;
-        r.size = size + 1;
-        r.exp = u.exp + 1;
-    } else {
-        for (int i = 0; i < L; ++i) r.t[i] = tmp[i];
-        r.size = size;
-        r.exp = u.exp;
-    }
+    const bool grow = c != 0;
+#pragma unroll
+    for (int i = L - 1; i >= 1; --i) r.t[i] = grow ? tmp[i - 1] : tmp[i];
+    r.t[0] = grow ? (limb)1 : tmp[0];
+    r.size = u.size ? size + (grow ? 1 : 0) : 0;
+    r.exp = u.size ? u.exp + (grow ? 1 : 0) : 0;
+    r.neg = u.size ? u.neg : 0;
 }
 
 /* mpf_abs in place */
@@ -186,15 +241,11 @@ GF_HD void gf_abs(gf<L>& r)
 template <int L>
 GF_HD bool gf_gt4(const gf<L>& x)
 {
-    if (x.size == 0 || x.neg)
-        return false;
-    if (x.exp != 1)
-        return x.exp > 1;
-    if (x.t[0] != 4)
-        return x.t[0] > 4;
     limb rest = 0;
-    for (int i = 1; i < L; ++i) rest |= (i < x.size) ? x.t[i] : 0;
-    return rest != 0;
+#pragma unroll
+    for (int i = 1; i < L; ++i) rest |= x.t[i];           /* zero beyond size */
+    const bool above = (x.exp > 1) | ((x.exp == 1) & ((x.t[0] > 4) | ((x.t[0] == 4) & (rest != 0))));
+    return (x.size != 0) & (x.neg == 0) & above;
 }
 
 template <int L>
@@ -223,199 +274,219 @@ GF_HD void gf_add(gf<L>& r, const gf<L>& a, const gf<L>& b, uint32_t flipb, int 
         return;
     }
     const bool sw = a.exp < b.exp;
-    const gf<L>& u = sw ? b : a;
-    const gf<L>& v = sw ? a : b;
-    const uint32_t negate = a.neg;
-    const int ediff = u.exp - v.exp;
-    int us = u.size, vs = v.size;
+    limb U[L], V[L];
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        U[i] = sw ? b.t[i] : a.t[i];
+        V[i] = sw ? a.t[i] : b.t[i];
+    }
+    const int uexp = sw ? b.exp : a.exp;
+    const int ediff = sw ? b.exp - a.exp : a.exp - b.exp;
+    int us = sw ? b.size : a.size, vs = sw ? a.size : b.size;
     if (us > P) us = P;
     if (vs + ediff > P) vs = P - ediff;
-    limb R[L + 1];
-    int rsize, exp = u.exp;
+    keep_top<L>(U, us);
+    int rsize;
     if (ediff >= P) {
-        for (int k = 0; k < L; ++k) R[k] = (k < us) ? u.t[k] : 0;
         rsize = us;
+#pragma unroll
+        for (int i = 0; i < L; ++i) V[i] = 0;
     } else {
-        if (us > ediff)
-            rsize = (vs + ediff <= us) ? us : vs + ediff;
-        else
-            rsize = vs + ediff;
-        limb c = 0;
-        for (int k = L - 1; k >= 0; --k) {
-            const limb x = (k < us) ? u.t[k] : 0;
-            const int j = k - ediff;
-            const limb y = (j >= 0 && j < vs) ? v.t[j] : 0;
-            R[k] = (k < rsize) ? addc(x, y, c, c) : 0;
-            if (k >= rsize) c = 0;
-        }
-        if (c) {
-            for (int k = L - 1; k >= 1; --k) R[k] = R[k - 1];
-            R[0] = 1;
-            ++rsize;
-            ++exp;
-        }
+        keep_top<L>(V, vs);
+        shift_down<L>(V, ediff);
+        rsize = (us > ediff) ? ((vs + ediff <= us) ? us : vs + ediff) : vs + ediff;
     }
-    for (int k = 0; k < L; ++k) r.t[k] = (k < rsize) ? R[k] : 0;
+    limb R[L];
+    limb c = 0;
+#pragma unroll
+    for (int k = L - 1; k >= 0; --k) R[k] = addc(U[k], V[k], c, c);
+    const bool grow = c != 0;
+#pragma unroll
+    for (int k = L - 1; k >= 1; --k) r.t[k] = grow ? R[k - 1] : R[k];
+    r.t[0] = grow ? (limb)1 : R[0];
+    rsize += grow ? 1 : 0;
+    keep_top<L>(r.t, rsize);
     r.size = rsize;
-    r.exp = rsize ? exp : 0;
-    r.neg = rsize ? negate : 0;
+    r.exp = rsize ? uexp + (grow ? 1 : 0) : 0;
+    r.neg = rsize ? a.neg : 0;
 }
 
 /* ------------------------------------------------------------------ */
-/* mpf_sub (mpf/sub.c) on magnitudes: r = sign(neg0) * (|u0| - |v0|).
-   Windows of u and v are (offset from the top of t, limb count).          */
+/* mpf_sub (mpf/sub.c) on magnitudes: r = sign(neg0) * (|u0| - |v0|).      */
+/* The windows of u and v are moved to the top of U and V as their leading */
+/* limbs are consumed.                                                      */
 /* ------------------------------------------------------------------ */
 template <int L>
 GF_HD void gf_sub_mag(gf<L>& r, const gf<L>& u0, const gf<L>& v0, uint32_t neg0, int lr)
 {
     const int prec = lr;         /* PREC(r) + 1 */
-    uint32_t negate = neg0;
-    const limb* up = u0.t;
-    const limb* vp = v0.t;
-    int uoff = 0, voff = 0;
-    int usize = u0.size, vsize = v0.size;
-    int exp = u0.exp;
-    int ediff = u0.exp - v0.exp;
-    if (u0.exp < v0.exp) {
-        up = v0.t;
-        vp = u0.t;
-        usize = v0.size;
-        vsize = u0.size;
-        exp = v0.exp;
-        ediff = -ediff;
-        negate ^= 1;
+    const bool sw = u0.exp < v0.exp;
+    uint32_t negate = neg0 ^ (sw ? 1u : 0u);
+    limb U[L], V[L], T[L];
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        U[i] = sw ? v0.t[i] : u0.t[i];
+        V[i] = sw ? u0.t[i] : v0.t[i];
+        T[i] = 0;
     }
-    limb T[L + 2];
-    for (int k = 0; k < L + 2; ++k) T[k] = 0;
+    int usize = sw ? v0.size : u0.size, vsize = sw ? u0.size : v0.size;
+    int exp = sw ? v0.exp : u0.exp;
+    const int ediff = sw ? v0.exp - u0.exp : u0.exp - v0.exp;
     int rsize = 0;
-    bool norm = true;            /* strip leading zero limbs at the end */
+    int mode = 0;      /* 0 general, 1 near-cancellation result, 2 copy of V, 3 copy of U */
+    bool norm = true;
+    bool one = false;  /* a leading limb 1 above T (near-cancellation, no borrow) */
 
-#define U(i) wl<L>(up, uoff, usize, (i))
-#define V(i) wl<L>(vp, voff, vsize, (i))
-    if (ediff <= 1) {
-        bool general = false;
-        if (ediff == 0) {
-            while (U(0) == V(0)) {
-                ++uoff; --usize; ++voff; --vsize; --exp;
-                if (usize == 0 || vsize == 0) {
-                    /* one operand cancels the other's top limbs: the rest of
-                       the longer one (high zero limbs stripped, cut to prec) */
-                    if (usize == 0) {
-                        negate ^= 1;
-                    } else {
-                        vp = up; voff = uoff; vsize = usize;
-                    }
-                    while (vsize != 0 && V(0) == 0) { ++voff; --vsize; --exp; }
-                    if (vsize > prec) vsize = prec;
-                    for (int k = 0; k < vsize; ++k) T[k] = V(k);
-                    rsize = vsize;
-                    goto done;
+    if (ediff == 0) {
+        bool done = false;
+        while (!done && U[0] == V[0]) {
+            drop_top<L>(U);
+            drop_top<L>(V);
+            --usize;
+            --vsize;
+            --exp;
+            if (usize == 0 || vsize == 0) {
+                /* one operand cancels the other's top limbs: the rest of the
+                   longer one, high zero limbs stripped, cut to prec */
+                if (usize == 0) {
+                    negate ^= 1;
+                } else {
+#pragma unroll
+                    for (int i = 0; i < L; ++i) V[i] = U[i];
+                    vsize = usize;
                 }
+                while (vsize != 0 && V[0] == 0) {
+                    drop_top<L>(V);
+                    --vsize;
+                    --exp;
+                }
+                if (vsize > prec) vsize = prec;
+                mode = 2;
+                done = true;
             }
-            if (U(0) < V(0)) {
-                const limb* tp = up; up = vp; vp = tp;
-                int t = uoff; uoff = voff; voff = t;
-                t = usize; usize = vsize; vsize = t;
+        }
+        if (mode == 0) {
+            if (U[0] < V[0]) {
+#pragma unroll
+                for (int i = 0; i < L; ++i) {
+                    const limb t = U[i];
+                    U[i] = V[i];
+                    V[i] = t;
+                }
+                const int t = usize;
+                usize = vsize;
+                vsize = t;
                 negate ^= 1;
             }
-            if (U(0) != V(0) + 1) {
-                general = true;
-            } else {
-                ++uoff; --usize; ++voff; --vsize; --exp;
-            }
-        } else {
-            if (U(0) != 1 || V(0) != LMAX || (usize >= 2 && U(1) != 0)) {
-                general = true;
-            } else {
-                ++uoff; --usize; --exp;
+            if (U[0] == V[0] + 1) {
+                drop_top<L>(U);
+                drop_top<L>(V);
+                --usize;
+                --vsize;
+                --exp;
+                mode = 1;
             }
         }
-        if (!general) {
-            while (vsize != 0 && usize != 0 && U(0) == 0 && V(0) == LMAX) {
-                ++uoff; --usize; ++voff; --vsize; --exp;
-            }
-            if (usize == 0) {
-                while (vsize != 0 && V(0) == LMAX) { ++voff; --vsize; --exp; }
-            } else if (usize > prec - 1) {
-                usize = prec - 1;
-            }
-            if (vsize > prec - 1) vsize = prec - 1;
-            if (vsize == 0) {
-                /* 1 followed by u's limbs */
-                T[0] = 1;
-                for (int k = 0; k < usize; ++k) T[k + 1] = U(k);
-                rsize = usize + 1;
-                ++exp;
-                norm = false;
-                goto store;
-            }
-            /* exact (1 . u) - (0 . v) on the aligned windows, both cut to
-               prec - 1 limbs: the difference of the windows with its borrow;
-               no borrow keeps the leading 1 */
-            const int n = usize > vsize ? usize : vsize;
-            limb bw = 0;
-            for (int k = n - 1; k >= 0; --k) T[k + 1] = subb(U(k), V(k), bw, bw);
-            rsize = n;
-            if (bw == 0) {
-                T[0] = 1;
-                ++rsize;
-                ++exp;
-            } else {
-                for (int k = 0; k < n; ++k) T[k] = T[k + 1];
-                T[n] = 0;
-            }
-            goto store;
+    } else if (ediff == 1) {
+        if (U[0] == 1 && V[0] == LMAX && !(usize >= 2 && U[1] != 0)) {
+            drop_top<L>(U);
+            --usize;
+            --exp;
+            mode = 1;
         }
     }
 
-    /* general case */
-    if (usize > prec) usize = prec;
-    if (vsize + ediff > prec) vsize = prec - ediff;
-    if (ediff >= prec) {
-        for (int k = 0; k < usize; ++k) T[k] = U(k);
-        rsize = usize;
-        norm = false;
-        goto store;
-    }
-    while (true) {
-        if (vsize <= 0) {
-            for (int k = 0; k < usize; ++k) T[k] = U(k);
-            rsize = usize;
-            goto done;
+    if (mode == 1) {
+        while (vsize != 0 && usize != 0 && U[0] == 0 && V[0] == LMAX) {
+            drop_top<L>(U);
+            drop_top<L>(V);
+            --usize;
+            --vsize;
+            --exp;
         }
-        if (V(vsize - 1) != 0) break;
-        --vsize;
-    }
-    while (true) {
         if (usize == 0) {
-            for (int k = 0; k < vsize; ++k) T[k] = V(k);
-            rsize = vsize;
-            negate ^= 1;
-            goto done;
+            while (vsize != 0 && V[0] == LMAX) {
+                drop_top<L>(V);
+                --vsize;
+                --exp;
+            }
+        } else if (usize > prec - 1) {
+            usize = prec - 1;
         }
-        if (U(usize - 1) != 0) break;
-        --usize;
-    }
-    {
-        /* |u| - |v| with v ediff limbs lower, exact over the union span */
-        rsize = (usize > ediff) ? ((vsize + ediff <= usize) ? usize : vsize + ediff)
-                                : vsize + ediff;
-        limb bw = 0;
-        for (int k = rsize - 1; k >= 0; --k) T[k] = subb(U(k), V(k - ediff), bw, bw);
+        if (vsize > prec - 1) vsize = prec - 1;
+        keep_top<L>(U, usize);
+        keep_top<L>(V, vsize);
+        if (vsize == 0) {
+            /* 1 followed by u's limbs */
+#pragma unroll
+            for (int k = 0; k < L; ++k) T[k] = U[k];
+            rsize = usize;
+            one = true;
+            norm = false;
+        } else {
+            /* (1 . u) - (0 . v) on the aligned windows, both cut to prec - 1
+               limbs: their difference, and no borrow keeps the leading 1 */
+            rsize = usize > vsize ? usize : vsize;
+            limb bw = 0;
+#pragma unroll
+            for (int k = L - 1; k >= 0; --k) T[k] = subb(U[k], V[k], bw, bw);
+            one = (bw == 0);
+        }
+    } else if (mode == 0) {
+        /* general case */
+        if (usize > prec) usize = prec;
+        if (vsize + ediff > prec) vsize = prec - ediff;
+        if (ediff >= prec) {
+            mode = 3;
+            norm = false;
+        } else {
+            keep_top<L>(U, usize);
+            keep_top<L>(V, vsize);
+            while (vsize > 0 && pick<L>(V, vsize - 1) == 0) --vsize;
+            if (vsize <= 0) {
+                mode = 3;
+                norm = false;
+            } else {
+                while (usize > 0 && pick<L>(U, usize - 1) == 0) --usize;
+                /* usize == 0 cannot happen: U's top limb is nonzero */
+                shift_down<L>(V, ediff);
+                rsize = (usize > ediff) ? ((vsize + ediff <= usize) ? usize : vsize + ediff)
+                                        : vsize + ediff;
+                limb bw = 0;
+#pragma unroll
+                for (int k = L - 1; k >= 0; --k) T[k] = subb(U[k], V[k], bw, bw);
+            }
+        }
     }
 
-store:
-    if (norm) {
-        int lead = 0;
-        while (lead < rsize && T[lead] == 0) ++lead;
-        for (int k = 0; k + lead < L + 2; ++k) T[k] = T[k + lead];
-        rsize -= lead;
-        exp -= lead;
+    if (mode == 2) {
+#pragma unroll
+        for (int k = 0; k < L; ++k) T[k] = V[k];
+        rsize = vsize;
+        norm = false;
+    } else if (mode == 3) {
+#pragma unroll
+        for (int k = 0; k < L; ++k) T[k] = U[k];
+        rsize = usize;
     }
-done:
-#undef U
-#undef V
-    for (int k = 0; k < L; ++k) r.t[k] = (k < rsize) ? T[k] : 0;
+    if (one) {
+        /* the leading 1 goes on top */
+#pragma unroll
+        for (int k = L - 1; k >= 1; --k) T[k] = T[k - 1];
+        T[0] = 1;
+        ++rsize;
+        ++exp;
+    }
+    keep_top<L>(T, rsize);
+    if (norm) {
+        while (rsize > 0 && T[0] == 0) {
+            drop_top<L>(T);
+            --rsize;
+            --exp;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < L; ++k) r.t[k] = T[k];
     r.size = rsize;
     r.exp = rsize ? exp : 0;
     r.neg = rsize ? negate : 0;

@@ -261,7 +261,8 @@ int main(int argc, char** argv)
     check<7>(cases);
     check<9>(cases);
     check<18>(cases / 4);
-    check<4, 6>(cases / 2);    // the device's storage buckets hold fewer limbs in use
+    check<3, 4>(cases / 2);    // the device's storage buckets hold fewer limbs in use
+    check<4, 6>(cases / 2);
     check<6, 10>(cases / 2);
     check<9, 18>(cases / 4);
     check_chain<3>(cases / 20);

@@ -73,7 +73,7 @@ def test_gmp_limb_precision_rule_matches_libgmp():
     class Mpf(ctypes.Structure):
         _fields_ = [("prec", ctypes.c_int), ("size", ctypes.c_int), ("exp", ctypes.c_long),
                     ("d", ctypes.c_void_p)]
-    buckets = (6, 10, 18)
+    buckets = (4, 6, 10, 18)
     for p in (2, 53, 54, 64, 80, 100, 128, 129, 192, 193, 256, 320, 500, 512, 640, 1000, 1024):
         x = Mpf()
         g.__gmpf_init2(ctypes.byref(x), ctypes.c_ulong(p))
