@@ -120,6 +120,52 @@ int mdzg_rect_from_center_coords(const char* cx, const char* cy, const char* siz
                                  long precision, int user_w, int user_h,
                                  char* xmin, char* xmax, char* ymax, char* width, int cap);
 
+/* ---- scene files --------------------------------------------------------
+   A fractal settings file (.mdz) as MDZ's command line reads its settings
+   section (image_info_load_settings, src/image_info.c:456-721, with
+   src/mdzfileio.c and src/setting.c). */
+typedef struct mdzg_scene {
+    int version_major, version_minor, version_revision;   /* header version    */
+    int family, fractal;            /* MDZG_FAMILY_*, MDZG_MANDELBROT..VARIANT    */
+    long long depth;
+    double aspect;                  /* width / height of the saved image         */
+    double colour_scale;
+    int colour_interpolate;
+    int multi_precision, rounding;  /* the file's flags ...                       */
+    int path;                       /* ... and the MDZG_PATH_* they select        */
+    long precision;                 /* >= 80                                      */
+    int centre;                     /* 1: c0..c2 = cx, cy, size; 0: xmin, xmax, ymax */
+    char* c0;                       /* the file's decimal strings                 */
+    char* c1;
+    char* c2;
+    char* julia_re;                 /* Julia family only                          */
+    char* julia_im;
+    int palette_offset;
+} mdzg_scene;
+
+int  mdzg_scene_load(const char* path, mdzg_scene* out);   /* MDZG_ERR_PARSE on a bad file */
+void mdzg_scene_free(mdzg_scene* s);
+
+/* The image size MDZ's command line renders a scene at (main.c init_misc):
+   *w, *h of 0 are filled in -- both 0 gives 480 x (int)(480 / aspect); one 0
+   follows the aspect 480 / (int)(480 / aspect) that loading stored. */
+int mdzg_scene_dims(const mdzg_scene* s, int* w, int* h);
+
+/* The rect (hex strings, as mdzg_rect_from_center gives them) MDZ's command
+   line renders a scene with at user_w x user_h.  Centre form: coords_to +
+   coords_size, then coords_center_to_rect -- mdzg_rect_from_center
+   (mdzg_rect_from_center_coords on the GMP path).  Corner form:
+   coords_set_rect + coords_rect_to_center at the aspect of the 480-wide load
+   (src/image_info.c:666-669, src/coords.c:327-333, :234-252); then init_misc's
+   coords_set puts coords' stale size, 4.0, back into the width (wide output)
+   or height (tall), and coords_center_to_rect (:265-313) centres that on the
+   file's rect.  flags = MDZG_SCENE_FIT_RECT instead keeps the file's width
+   and refits its rect at the output aspect (no counterpart in the
+   reference's command line). */
+enum { MDZG_SCENE_FIT_RECT = 1 };
+int mdzg_scene_rect(const mdzg_scene* s, int user_w, int user_h, int flags,
+                    char* xmin, char* xmax, char* ymax, char* width, int cap);
+
 /* Synchronous render of one tile: what fractal_*_calculate_line computes for
    those rows and columns (src/fractal.c:29-257), in one persistent launch. */
 int mdzg_render_tile(mdzg_ctx* ctx, const mdzg_tile* tile, mdzg_stats* stats);

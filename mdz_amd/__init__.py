@@ -52,6 +52,20 @@ class Tile(ctypes.Structure):
                 ("counts", ctypes.POINTER(ctypes.c_uint32))]
 
 
+class Scene(ctypes.Structure):
+    _fields_ = [
+        ("version_major", ctypes.c_int), ("version_minor", ctypes.c_int),
+        ("version_revision", ctypes.c_int), ("family", ctypes.c_int), ("fractal", ctypes.c_int),
+        ("depth", ctypes.c_longlong), ("aspect", ctypes.c_double),
+        ("colour_scale", ctypes.c_double), ("colour_interpolate", ctypes.c_int),
+        ("multi_precision", ctypes.c_int), ("rounding", ctypes.c_int), ("path", ctypes.c_int),
+        ("precision", ctypes.c_long), ("centre", ctypes.c_int),
+        ("c0", ctypes.c_char_p), ("c1", ctypes.c_char_p), ("c2", ctypes.c_char_p),
+        ("julia_re", ctypes.c_char_p), ("julia_im", ctypes.c_char_p),
+        ("palette_offset", ctypes.c_int),
+    ]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
                 ("pixel_iters", ctypes.c_double), ("launches", ctypes.c_int)]
@@ -107,6 +121,12 @@ def lib():
         L.mdzg_rect_from_center.argtypes = [ctypes.c_char_p] * 3 + [
             ctypes.c_long, ctypes.c_int, ctypes.c_int] + [ctypes.c_char_p] * 4 + [ctypes.c_int]
         L.mdzg_rect_from_center_coords.argtypes = L.mdzg_rect_from_center.argtypes
+        L.mdzg_scene_load.argtypes = [ctypes.c_char_p, P(Scene)]
+        L.mdzg_scene_free.argtypes = [P(Scene)]
+        L.mdzg_scene_free.restype = None
+        L.mdzg_scene_dims.argtypes = [P(Scene), P(ctypes.c_int), P(ctypes.c_int)]
+        L.mdzg_scene_rect.argtypes = [P(Scene), ctypes.c_int, ctypes.c_int, ctypes.c_int] + \
+            [ctypes.c_char_p] * 4 + [ctypes.c_int]
         L.mdzg_render_tile.argtypes = [vp, P(Tile), P(Stats)]
         L.mdzg_render_batch.argtypes = [vp, P(Tile), ctypes.c_int, ctypes.c_int, P(Stats)]
         L.mdzg_submit.argtypes = [vp, P(Tile), ctypes.c_int, P(vp)]
@@ -163,6 +183,47 @@ def rect_from_center(cx, cy, size, precision, user_w, user_h, coords=False):
 def kernel_id(path, precision):
     """Id of the kernel a view on (path, precision) runs on."""
     return _check(lib().mdzg_kernel_id(path, precision), "kernel_id")
+
+
+def load_scene(path):
+    """The settings of a .mdz file (mdzg_scene_load) as a dict."""
+    sc = Scene()
+    _check(lib().mdzg_scene_load(_b(path), ctypes.byref(sc)), "scene_load")
+    try:
+        d = {k: getattr(sc, k) for k, _ in Scene._fields_}
+        for k in ("c0", "c1", "c2", "julia_re", "julia_im"):
+            d[k] = d[k].decode() if d[k] is not None else None
+    finally:
+        lib().mdzg_scene_free(ctypes.byref(sc))
+    return d
+
+
+SCENE_FIT_RECT = 1
+
+
+def scene_view(path, width=0, height=0, aa=1, fit_rect=False):
+    """View of a .mdz file as MDZ's command line renders it: width x height
+    (0 = as main.c's init_misc fills it in, mdzg_scene_dims) with aa x aa
+    samples per pixel, on the rect of mdzg_scene_rect (fit_rect: a
+    corner-form file's own rect refitted, MDZG_SCENE_FIT_RECT)."""
+    sc = Scene()
+    _check(lib().mdzg_scene_load(_b(path), ctypes.byref(sc)), "scene_load")
+    try:
+        w, h = ctypes.c_int(width), ctypes.c_int(height)
+        _check(lib().mdzg_scene_dims(ctypes.byref(sc), ctypes.byref(w), ctypes.byref(h)),
+               "scene_dims")
+        cap = 4096
+        bufs = [ctypes.create_string_buffer(cap) for _ in range(4)]
+        _check(lib().mdzg_scene_rect(ctypes.byref(sc), w.value, h.value,
+                                     SCENE_FIT_RECT if fit_rect else 0, *bufs, cap), "scene_rect")
+        rect = tuple(b.value.decode() for b in bufs)
+        julia = (sc.julia_re.decode(), sc.julia_im.decode()) if sc.julia_re else (None, None)
+        v = make_view(rect, precision=sc.precision, depth=sc.depth, width=w.value * aa,
+                      height=h.value * aa, fractal=sc.fractal, family=sc.family, path=sc.path,
+                      julia=julia)
+    finally:
+        lib().mdzg_scene_free(ctypes.byref(sc))
+    return v
 
 
 def make_view(rect, *, precision, depth, width, height, fractal=MANDELBROT,

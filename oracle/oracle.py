@@ -36,7 +36,7 @@ _lib = None
 def build():
     """Compile liboracle.so (and oracle/_ref when the reference tree exists)."""
     import subprocess
-    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", HERE, "all"], check=True)
 
 
 def lib():

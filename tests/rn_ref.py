@@ -133,3 +133,51 @@ def rect_from_center(cx, cy, size, p, user_w, user_h, coords=False):
     if coords:
         return (xmin, xmax, ymax, width)
     return tuple(rn(v, hp) for v in (xmin, xmax, ymax, width))
+
+
+def rect_from_corners(xmin, xmax, ymax, p, user_w, user_h, coords=False, file_aspect=None):
+    """A corner-form settings file's rect at user_w x user_h, exact RN
+    arithmetic at the coords precision (coords=True keeps that precision).
+
+    file_aspect given: as MDZ's command line renders it -- the centre from
+    coords_set_rect / coords_rect_to_center (src/coords.c:327-333, :234-252)
+    at the aspect of the 480-wide load, 480 / int(480 / file_aspect), then the
+    stale size 4.0 that init_misc's coords_set restores (:255-262) as the width
+    (wide) or height (tall), then coords_center_to_rect (:265-313).
+    file_aspect None: the file's rect refitted at the output aspect."""
+    cp = max(p, 80)
+    cp = max(cp, gmp_mpf_bits(cp))
+    hp = max(p, 80)
+    ad = user_w / user_h
+    aspect = Fraction(ad)
+    aspect0 = aspect if file_aspect is None else Fraction(480 / int(480 / file_aspect))
+    xmin, xmax, ymax = (rn(rn(parse(s), p), cp) for s in (xmin, xmax, ymax))
+    width = rn(xmax - xmin, cp)
+    height = rn(width / aspect0, cp)
+    ymin = rn(ymax - height, cp)
+    cx = rn(rn(xmin + xmax, cp) / 2, cp)
+    cy = rn(rn(ymin + ymax, cp) / 2, cp)
+    if file_aspect is not None:
+        if ad > 1.0:
+            width = Fraction(4)
+        else:
+            height = Fraction(4)
+    if ad > 1.0:
+        height = rn(width / aspect, cp)
+        tmp = rn(width / 2, cp)
+        xmin = rn(cx - tmp, cp)
+        xmax = rn(xmin + width, cp)
+        tmp = rn(tmp / aspect, cp)
+        ymin = rn(cy - tmp, cp)
+        ymax = rn(ymin + height, cp)
+    else:
+        width = rn(height * aspect, cp)
+        tmp = rn(height / 2, cp)
+        ymin = rn(cy - tmp, cp)
+        ymax = rn(ymin + height, cp)
+        tmp = rn(tmp * aspect, cp)
+        xmin = rn(cx - tmp, cp)
+        xmax = rn(xmin + width, cp)
+    if coords:
+        return (xmin, xmax, ymax, width)
+    return tuple(rn(v, hp) for v in (xmin, xmax, ymax, width))

@@ -62,6 +62,25 @@ def test_gallery_scenes(ctx, oracle, name):
     assert _eq(got, ref)
 
 
+@pytest.mark.parametrize("name", sorted(cases.SCENES))
+def test_scene_files(ctx, oracle, name):
+    """A gallery .mdz file loaded and rendered as MDZ's command line renders it
+    (its own path, its rect at 240 wide; corner-form files also refitted),
+    against the oracle on that rect."""
+    import os
+    f = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mdz", name + ".mdz")
+    for fit in ((False, True) if cases.SCENES[name][0] == "corner" else (False,)):
+        v = M.scene_view(f, 240, fit_rect=fit)
+        W, H = v.real_width, v.real_height
+        lines, cols = (H // 3, min(6, H // 2)), (W // 4, 48)
+        print(name, fit, v.path, W, H, flush=True)
+        got, _ = ctx.render(v, lines=lines, cols=cols)
+        ref = oracle.render(_rect(v), precision=v.precision, depth=v.depth, width=W, height=H,
+                            fractal=v.fractal, path=v.path, lines=(lines[0], lines[1], 1),
+                            cols=cols, threads=16)
+        assert _eq(got, ref), (name, fit)
+
+
 @pytest.mark.parametrize("p", [2, 17, 32, 33, 53, 63, 64, 65, 80, 96, 100, 127, 128, 129,
                                176, 184, 192, 255, 256, 320, 384, 500, 512, 640, 768, 1000, 1024])
 def test_every_precision_class(ctx, oracle, p):
