@@ -37,7 +37,7 @@ PEAK_TOPS = 78.6
 CONFIG2_TRAFFIC_BYTES = 3.07e6
 # VALU instructions (wave64) per config 2 launch of k_rn_persist<4,true>:
 # rocprofv3 SQ_INSTS_VALU of the current kernel build
-CONFIG2_VALU_INSTS = 9.3233e9
+CONFIG2_VALU_INSTS = 8.8672e9
 # VALU issue limit: 256 CUs x one wave64 instruction per clock x 2.4 GHz peak clock
 PEAK_VALU_ISSUE = 256 * 2.4e9
 
@@ -353,7 +353,8 @@ def main():
                    roofline=dict(bound="valu", achieved=achieved, peak=PEAK_TOPS, unit="TOP/s",
                                  frac=achieved / PEAK_TOPS, traffic=CONFIG2_TRAFFIC_BYTES,
                                  kernel_ms=kavg,
-                                 ops="36 MAC (x2 ops) per pixel-iteration x 1.536e9 per launch",
+                                 ops="2n^2+n = 36 algorithmic 32x32-bit MACs (x2 ops) per pixel-iteration (n = 4) "
+                                     "x 1.536e9 per launch",
                                  valu_issue=dict(
                                      insts_per_launch=CONFIG2_VALU_INSTS,
                                      achieved=CONFIG2_VALU_INSTS / (kavg * 1e-3),
