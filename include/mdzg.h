@@ -120,6 +120,42 @@ int mdzg_rect_from_center_coords(const char* cx, const char* cy, const char* siz
                                  long precision, int user_w, int user_h,
                                  char* xmin, char* xmax, char* ymax, char* width, int cap);
 
+/* ---- colouring -----------------------------------------------------------
+   A palette as MDZ's colouring reads it: the colours (palette[], RGB(r, g, b)
+   = r | g << 8 | b << 16, src/palette.h:16), their count pal_indexes, the
+   rotation pal_offset, and image_info's colour_scale and palette_ip. */
+typedef struct mdzg_palette {
+    uint32_t colours[256];
+    int n;                      /* pal_indexes, 2..256 (MDZ indexes modulo n - 1) */
+    int offset;                 /* pal_offset, >= 0                               */
+    double colour_scale;        /* > 0                                            */
+    int interpolate;            /* palette_ip                                     */
+} mdzg_palette;
+
+/* A palette file (.map): palette_load / palette_read (src/palette.c:167-217),
+   up to 256 lines of "r g b".  Sets colours and n; keeps the other fields. */
+int mdzg_palette_load(const char* path, mdzg_palette* pal);
+
+/* Colour a frame of counts, as render_to_file does row by row: palette_apply
+   (src/palette.c:406-463) when aa == 1, do_anti_aliasing (src/render.c:108-157)
+   when aa > 1.  counts: (user_w * aa) x (user_h * aa) samples, MDZ's raw_data;
+   rgb: user_w x user_h colours, MDZ's rgb_data.  On the device; stats (may be
+   NULL) times the colouring kernel alone. */
+int mdzg_colour(mdzg_ctx* ctx, const uint32_t* counts, int user_w, int user_h, int aa,
+                const mdzg_palette* pal, uint32_t* rgb, mdzg_stats* stats);
+
+/* Render a whole view (real_width x real_height = user size x aa) and colour
+   it on the device in one job: the counts stay in device memory and only the
+   colours come back (and the counts, when counts is not NULL).  stats counts
+   the frame's pixel-iterations as mdzg_render_tile does. */
+int mdzg_render_image(mdzg_ctx* ctx, const mdzg_view* view, int aa, const mdzg_palette* pal,
+                      uint32_t* rgb, uint32_t* counts, mdzg_stats* stats);
+
+/* save_png_file (src/my_png.c:28-109): an 8-bit RGB, non-interlaced PNG of
+   w x h rgb_data colours (the byte above blue is dropped, as png_set_filler
+   does).  MDZG_ERR_ARG if the file cannot be written. */
+int mdzg_png_write(const char* path, const uint32_t* rgb, int w, int h);
+
 /* ---- scene files --------------------------------------------------------
    A fractal settings file (.mdz) as MDZ's command line reads its settings
    section (image_info_load_settings, src/image_info.c:456-721, with
@@ -141,10 +177,17 @@ typedef struct mdzg_scene {
     char* julia_re;                 /* Julia family only                          */
     char* julia_im;
     int palette_offset;
+    int palette_n;                  /* colours of the palette section: its "data" */
+    uint32_t palette[256];          /*   lines or the "file" it names; 0 if none or unreadable */
 } mdzg_scene;
 
 int  mdzg_scene_load(const char* path, mdzg_scene* out);   /* MDZG_ERR_PARSE on a bad file */
 void mdzg_scene_free(mdzg_scene* s);
+
+/* The scene's palette (its palette section, palette-offset, colour-scale and
+   colour-interpolate).  MDZG_ERR_PARSE when the file has no usable palette,
+   a file MDZ's command line refuses (image_info_load_all, src/image_info.c:769-783). */
+int mdzg_scene_palette(const mdzg_scene* s, mdzg_palette* pal);
 
 /* The image size MDZ's command line renders a scene at (main.c init_misc):
    *w, *h of 0 are filled in -- both 0 gives 480 x (int)(480 / aspect); one 0

@@ -63,7 +63,14 @@ class Scene(ctypes.Structure):
         ("c0", ctypes.c_char_p), ("c1", ctypes.c_char_p), ("c2", ctypes.c_char_p),
         ("julia_re", ctypes.c_char_p), ("julia_im", ctypes.c_char_p),
         ("palette_offset", ctypes.c_int),
+        ("palette_n", ctypes.c_int), ("palette", ctypes.c_uint32 * 256),
     ]
+
+
+class Palette(ctypes.Structure):
+    _fields_ = [("colours", ctypes.c_uint32 * 256), ("n", ctypes.c_int),
+                ("offset", ctypes.c_int), ("colour_scale", ctypes.c_double),
+                ("interpolate", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -125,6 +132,14 @@ def lib():
         L.mdzg_scene_free.argtypes = [P(Scene)]
         L.mdzg_scene_free.restype = None
         L.mdzg_scene_dims.argtypes = [P(Scene), P(ctypes.c_int), P(ctypes.c_int)]
+        L.mdzg_scene_palette.argtypes = [P(Scene), P(Palette)]
+        L.mdzg_palette_load.argtypes = [ctypes.c_char_p, P(Palette)]
+        L.mdzg_colour.argtypes = [vp, P(ctypes.c_uint32), ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, P(Palette), P(ctypes.c_uint32), P(Stats)]
+        L.mdzg_render_image.argtypes = [vp, P(View), ctypes.c_int, P(Palette),
+                                        P(ctypes.c_uint32), P(ctypes.c_uint32), P(Stats)]
+        L.mdzg_png_write.argtypes = [ctypes.c_char_p, P(ctypes.c_uint32), ctypes.c_int,
+                                     ctypes.c_int]
         L.mdzg_scene_rect.argtypes = [P(Scene), ctypes.c_int, ctypes.c_int, ctypes.c_int] + \
             [ctypes.c_char_p] * 4 + [ctypes.c_int]
         L.mdzg_render_tile.argtypes = [vp, P(Tile), P(Stats)]
@@ -185,12 +200,57 @@ def kernel_id(path, precision):
     return _check(lib().mdzg_kernel_id(path, precision), "kernel_id")
 
 
+def make_palette(colours, *, offset=0, colour_scale=1.0, interpolate=False):
+    """A Palette from a sequence of RGB(r, g, b) colours (r | g << 8 | b << 16)
+    or of (r, g, b) triples."""
+    p = Palette()
+    cols = [c if isinstance(c, int) else (c[0] | (c[1] << 8) | (c[2] << 16)) for c in colours]
+    if not 2 <= len(cols) <= 256:
+        raise ValueError("a palette has 2..256 colours")
+    for i, c in enumerate(cols):
+        p.colours[i] = c & 0xffffffff
+    p.n, p.offset, p.colour_scale, p.interpolate = len(cols), offset, colour_scale, int(interpolate)
+    return p
+
+
+def load_palette(path, *, offset=0, colour_scale=1.0, interpolate=False):
+    """A .map palette file (mdzg_palette_load)."""
+    p = make_palette([0, 0], offset=offset, colour_scale=colour_scale, interpolate=interpolate)
+    _check(lib().mdzg_palette_load(_b(path), ctypes.byref(p)), "palette_load")
+    return p
+
+
+def scene_palette(path):
+    """The palette of a .mdz file with its offset, colour scale and interpolation."""
+    sc = Scene()
+    _check(lib().mdzg_scene_load(_b(path), ctypes.byref(sc)), "scene_load")
+    try:
+        p = Palette()
+        _check(lib().mdzg_scene_palette(ctypes.byref(sc), ctypes.byref(p)), "scene_palette")
+    finally:
+        lib().mdzg_scene_free(ctypes.byref(sc))
+    return p
+
+
+def palette_colours(p):
+    return np.ctypeslib.as_array(p.colours)[:p.n].copy()
+
+
+def png_write(path, rgb):
+    """Write an [h, w] uint32 rgb_data array as an 8-bit RGB PNG (mdzg_png_write)."""
+    a = np.ascontiguousarray(rgb, dtype=np.uint32)
+    h, w = a.shape
+    _check(lib().mdzg_png_write(_b(path), a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                w, h), "png_write")
+
+
 def load_scene(path):
     """The settings of a .mdz file (mdzg_scene_load) as a dict."""
     sc = Scene()
     _check(lib().mdzg_scene_load(_b(path), ctypes.byref(sc)), "scene_load")
     try:
         d = {k: getattr(sc, k) for k, _ in Scene._fields_}
+        d["palette"] = np.ctypeslib.as_array(sc.palette)[:sc.palette_n].copy()
         for k in ("c0", "c1", "c2", "julia_re", "julia_im"):
             d[k] = d[k].decode() if d[k] is not None else None
     finally:
@@ -269,6 +329,34 @@ class Context:
         st = Stats()
         _check(lib().mdzg_render_tile(self._c, ctypes.byref(t), ctypes.byref(st)), "render")
         return out, st.as_dict()
+
+    def colour(self, counts, aa, palette, stats=False):
+        """rgb_data [h, w] uint32 of a [h*aa, w*aa] count array (mdzg_colour);
+        with stats=True also the stats of the colouring kernel."""
+        c = np.ascontiguousarray(counts, dtype=np.uint32)
+        rh, rw = c.shape
+        if rw % aa or rh % aa:
+            raise ValueError("count array is not a whole number of aa x aa blocks")
+        out = np.empty((rh // aa, rw // aa), dtype=np.uint32)
+        P = ctypes.POINTER(ctypes.c_uint32)
+        st = Stats()
+        _check(lib().mdzg_colour(self._c, c.ctypes.data_as(P), rw // aa, rh // aa, aa,
+                                 ctypes.byref(palette), out.ctypes.data_as(P),
+                                 ctypes.byref(st)), "colour")
+        return (out, st.as_dict()) if stats else out
+
+    def render_image(self, view, aa, palette, counts=False):
+        """Render a whole view and colour it on the device (mdzg_render_image):
+        rgb_data [real_height/aa, real_width/aa] (and the counts when asked), stats."""
+        P = ctypes.POINTER(ctypes.c_uint32)
+        rgb = np.empty((view.real_height // aa, view.real_width // aa), dtype=np.uint32)
+        cnt = np.empty((view.real_height, view.real_width), dtype=np.uint32) if counts else None
+        st = Stats()
+        _check(lib().mdzg_render_image(self._c, ctypes.byref(view), aa, ctypes.byref(palette),
+                                       rgb.ctypes.data_as(P),
+                                       cnt.ctypes.data_as(P) if counts else None,
+                                       ctypes.byref(st)), "render_image")
+        return (rgb, cnt, st.as_dict()) if counts else (rgb, st.as_dict())
 
     def render_batch(self, specs, mode=BATCH_PERSISTENT):
         """specs: list of (view, lines, cols).  Returns ([counts...], stats)."""

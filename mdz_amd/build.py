@@ -35,7 +35,7 @@ C_FLAGS = [
     "-ffp-contract=off", "-pthread",
     "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{MP_PREFIX}/include",
 ]
-LINK_LIBS = [f"-L{MP_PREFIX}/lib", f"-Wl,-rpath,{MP_PREFIX}/lib", "-lmpfr", "-lgmp",
+LINK_LIBS = [f"-L{MP_PREFIX}/lib", f"-Wl,-rpath,{MP_PREFIX}/lib", "-lmpfr", "-lgmp", "-lz",
              "-lpthread", "-lm"]
 
 
@@ -55,6 +55,8 @@ def jobs():
                    "-o", os.path.join(BUILD, "k_gmp.o")], "k_gmp.o"))
     steps.append(([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "k_fp64.hip"),
                    "-o", os.path.join(BUILD, "k_fp64.o")], "k_fp64.o"))
+    steps.append(([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "k_colour.hip"),
+                   "-o", os.path.join(BUILD, "k_colour.o")], "k_colour.o"))
     for n in sorted(NWORDS, reverse=True):   # largest (slowest) first
         steps.append(([hipcc, *HIP_FLAGS, f"-DMDZG_N={n}", "-c", os.path.join(CSRC, "k_rn.hip"),
                        "-o", os.path.join(BUILD, f"k_rn{n}.o")], f"k_rn{n}.o"))

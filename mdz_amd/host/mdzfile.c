@@ -37,7 +37,8 @@ typedef struct reader {
     int has_back;
 } reader;
 
-/* next non-comment line, tabs as spaces, cut at the first control character */
+/* next non-comment line, tabs as spaces, cut at the first control character
+   (or byte >= 0x80) */
 static int next_line(reader* r)
 {
     do {
@@ -50,7 +51,7 @@ static int next_line(reader* r)
             for (char* p = r->line; *p; ++p) {
                 if (*p == '\t') {
                     *p = ' ';
-                } else if ((unsigned char)*p < ' ') {
+                } else if ((signed char)*p < ' ') {   /* bytes >= 0x80 too, as char is signed */
                     *p = '\0';
                     break;
                 }
@@ -127,6 +128,64 @@ static char* get_number(const char* buf, const char* name)
     while (*s == ' ')
         ++s;
     return strdup(s);
+}
+
+/* mdzfile_test_for_name (src/mdzfileio.c:66-86): the line names `name` and
+   has room for a value */
+static int test_name(const char* buf, const char* name)
+{
+    const size_t nlen = strlen(name);
+    return !(strlen(buf) < nlen + 2 || strncmp(name, buf, nlen));
+}
+
+/* palette_read (src/palette.c:187-217): up to 256 lines " r g b" read raw,
+   stopping at the first that is not three integers; the colour count, 0 if
+   none.  RGB(r, g, b) with two's complement shifts, as MDZ builds it. */
+static int palette_read(FILE* f, uint32_t* colours)
+{
+    char buf[256];                                   /* BUF_SIZE, src/palette.c:8 */
+    int i = 0;
+    while (fgets(buf, sizeof buf, f)) {
+        if (i >= 256)
+            break;
+        int rr, gg, bb;
+        if (sscanf(buf, " %d %d %d", &rr, &gg, &bb) != 3)
+            break;
+        colours[i++] = (uint32_t)rr | ((uint32_t)gg << 8) | ((uint32_t)bb << 16);
+    }
+    return i;
+}
+
+int mdzg_palette_load(const char* path, mdzg_palette* pal)
+{
+    if (!path || !pal)
+        return MDZG_ERR_ARG;
+    FILE* f = fopen(path, "r");
+    if (!f)
+        return MDZG_ERR_ARG;
+    uint32_t c[256];
+    const int n = palette_read(f, c);
+    fclose(f);
+    if (n == 0)
+        return MDZG_ERR_PARSE;
+    memcpy(pal->colours, c, sizeof(uint32_t) * (size_t)n);
+    pal->n = n;
+    return MDZG_OK;
+}
+
+int mdzg_scene_palette(const mdzg_scene* s, mdzg_palette* pal)
+{
+    if (!s || !pal)
+        return MDZG_ERR_ARG;
+    if (s->palette_n < 1)
+        return MDZG_ERR_PARSE;
+    memset(pal, 0, sizeof *pal);
+    memcpy(pal->colours, s->palette, sizeof(uint32_t) * (size_t)s->palette_n);
+    pal->n = s->palette_n;
+    pal->offset = s->palette_offset;
+    pal->colour_scale = s->colour_scale;
+    pal->interpolate = s->colour_interpolate;
+    return MDZG_OK;
 }
 
 static const char* const opt_no_yes[] = { "no", "yes", NULL };
@@ -250,13 +309,51 @@ int mdzg_scene_load(const char* path, mdzg_scene* s)
         if (!next_line(&r) || !(s->julia_im = get_number(r.line, "julia-imag")))
             goto out;
     }
-    /* optional palette offset (image_info.c:601-614): a line read ahead */
-    if (next_line(&r)) {
-        if (name_ok(r.line, "palette-offset") && strlen(r.line) >= strlen("palette-offset") + 2) {
-            long off;
-            if (!get_long(r.line, "palette-offset", &off, 0, 255))
+    /* optional palette offset and random-palette settings (image_info.c:601-663),
+       each tested on a line read ahead (mdzfile_test_for_name) */
+    int have = next_line(&r);
+    if (have && test_name(r.line, "palette-offset")) {
+        long off;
+        if (!get_long(r.line, "palette-offset", &off, 0, 255))
+            goto out;
+        s->palette_offset = (int)off;
+        have = next_line(&r);
+    }
+    if (have && test_name(r.line, "r-strength")) {
+        static const char* const rnd_d[] = { "r-strength", "r-bands", "g-strength", "g-bands",
+                                             "b-strength", "b-bands" };
+        static const char* const rnd_l[] = { "rnd-offset", "rnd-stripe", "rnd-spread" };
+        for (int i = 0; i < 6; ++i) {
+            double v;
+            if ((i && !next_line(&r)) || !get_double(r.line, rnd_d[i], &v, 0.0, 1.0))
                 goto out;
-            s->palette_offset = (int)off;
+        }
+        for (int i = 0; i < 3; ++i) {
+            long v;
+            if (!next_line(&r) || !get_long(r.line, rnd_l[i], &v, 0, 255))
+                goto out;
+        }
+        have = next_line(&r);
+    }
+    /* the palette section (image_info_load_all, image_info.c:769-783, and
+       image_info_load_palette, :724-740): "palette", then "data" and the
+       colours, or "file <path>".  MDZ's command line refuses a file whose
+       palette is missing or unreadable; here the settings still load, with
+       palette_n = 0 */
+    while (have && strcmp(r.line, "palette") != 0)
+        have = next_line(&r);
+    if (have && next_line(&r)) {
+        if (strcmp(r.line, "data") == 0) {
+            s->palette_n = palette_read(r.f, s->palette);
+        } else if (test_name(r.line, "file")) {
+            const char* fn = r.line + strlen("file");
+            while (*fn == ' ')
+                ++fn;
+            FILE* pf = fopen(fn, "r");
+            if (pf) {
+                s->palette_n = palette_read(pf, s->palette);
+                fclose(pf);
+            }
         }
     }
     rc = MDZG_OK;

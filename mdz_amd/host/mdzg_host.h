@@ -26,6 +26,26 @@ int mdzg_dev_occupancy_slice(int kid);
 int mdzg_dev_launch_slice(int kid, const mdzg_dtile* d_tiles, int ntiles, uint64_t total,
                           unsigned long long* d_counter, int grid, int budget, hipStream_t st);
 
+/* k_colour.hip: counts -> RGB (palette_apply / do_anti_aliasing), adding the
+   pixel-iterations of the counts read to *iters when iters is not null */
+int mdzg_dev_launch_colour(const uint32_t* counts, int real_w, int user_w, int user_h, int aa,
+                           const uint32_t* palette, int n, int offset, double sc, int ip,
+                           uint32_t depth, uint32_t* rgb, unsigned long long* iters,
+                           hipStream_t st);
+
+/* An image job: the one tile of the job is a whole frame of user_w*aa x
+   user_h*aa samples, coloured on the device after its escape-time kernels. */
+typedef struct mdzg_image_req {
+    int user_w, user_h, aa;
+    const mdzg_palette* pal;
+    uint32_t* rgb;              /* user_w x user_h colours out */
+    uint32_t* counts;           /* the frame's counts out, or NULL */
+} mdzg_image_req;
+int mdzg_rt_render_image(mdzg_ctx* c, const mdzg_tile* tile, const mdzg_image_req* img,
+                         mdzg_stats* st);
+/* validation shared by the colouring entry points */
+int mdzg_palette_ok(const mdzg_palette* p);
+
 extern const int mdzg_nwords[MDZG_NCLASS_N];
 extern const int mdzg_gmp_limbs[MDZG_NGMP];
 

@@ -68,6 +68,9 @@ struct mdzg_job {
     int rc;
     int launches;
     int cancelled;
+    /* image jobs (mdzg_render_image): colour the frame on the device */
+    const mdzg_image_req* img;
+    size_t off_pal, off_iters, off_rgb, rgb_bytes;
 };
 
 double mdzg_now_ms(void)
@@ -288,7 +291,8 @@ static void job_free_mem(mdzg_job* j)
     free(j);
 }
 
-static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int mode, mdzg_job** out)
+static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int mode,
+                          const mdzg_image_req* img, mdzg_job** out)
 {
     if (!c || !tiles || ntiles < 1 || !out || (mode != MDZG_BATCH_PERSISTENT && mode != MDZG_BATCH_PER_TILE))
         return MDZG_ERR_ARG;
@@ -303,6 +307,7 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
     j->ctx = c;
     j->slot = -1;
     j->mode = mode;
+    j->img = img;
     j->ntiles = ntiles;
     j->tiles = (mdzg_tile*)malloc(sizeof(mdzg_tile) * (size_t)ntiles);
     j->out_off = (size_t*)malloc(sizeof(size_t) * (size_t)ntiles);
@@ -347,6 +352,10 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
     off = ALIGN_UP(sizeof(mdzg_dtile) * (size_t)ntiles, 256);
     j->off_ctr = off;
     off = ALIGN_UP(off + 8 * (size_t)(j->nkid + 1), 256);   /* counters + cancel flag */
+    j->off_iters = off;                                      /* image: pixel-iteration sum */
+    j->off_pal = off + 256;                                  /* image: palette */
+    if (img)
+        off += 256 + 4 * 256;
     j->off_coords = off;
     for (int i = 0; i < ntiles; ++i) {
         if (pv[i].vs) {
@@ -360,8 +369,14 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
         j->out_off[i] = off;
         off += ALIGN_UP((size_t)tiles[i].nlines * (size_t)tiles[i].ncols * 4, 16);
     }
+    off = ALIGN_UP(off, 256);
+    j->out_bytes = off - j->off_out;
+    j->off_rgb = off;
+    if (img) {
+        j->rgb_bytes = 4 * (size_t)img->user_w * (size_t)img->user_h;
+        off += j->rgb_bytes;
+    }
     j->bytes = ALIGN_UP(off, 256);
-    j->out_bytes = j->bytes - j->off_out;
 
     j->slot = acquire(c, j->bytes, &j->d, &j->h);
     if (j->slot < 0) {
@@ -383,6 +398,8 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
         char* H = (char*)j->h;
         char* D = (char*)j->d;
         memset(H, 0, j->off_out);
+        if (img)
+            memcpy(H + j->off_pal, img->pal->colours, 4 * (size_t)img->pal->n);
         mdzg_dtile* desc = (mdzg_dtile*)(H + off_desc);
         size_t coff = j->off_coords;
         uint64_t pix_begin = 0;
@@ -510,9 +527,34 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
         if (fork)
             for (int k = 1; k < j->nkid; ++k)
                 hipStreamWaitEvent(j->stream, j->ev_cls[k], 0);
+        if (img) {
+            /* one tile, the whole frame: colour it where it lies */
+            const mdzg_palette* P = img->pal;
+            if (mdzg_dev_launch_colour((const uint32_t*)(D + j->out_off[0]), tiles[0].ncols,
+                                       img->user_w, img->user_h, img->aa,
+                                       (const uint32_t*)(D + j->off_pal), P->n, P->offset,
+                                       P->colour_scale, P->interpolate,
+                                       (uint32_t)tiles[0].view.depth, (uint32_t*)(D + j->off_rgb),
+                                       (unsigned long long*)(D + j->off_iters), j->stream) != 0) {
+                rc = MDZG_ERR_DEVICE;
+                goto done;
+            }
+            j->launches++;
+        }
         hipEventRecord(j->ev1, j->stream);
-        if (hipMemcpyAsync((char*)j->h + j->off_out, D + j->off_out, j->out_bytes,
-                           hipMemcpyDeviceToHost, j->stream) != hipSuccess) {
+        if (img) {
+            if (hipMemcpyAsync((char*)j->h + j->off_iters, D + j->off_iters, 8,
+                               hipMemcpyDeviceToHost, j->stream) != hipSuccess ||
+                hipMemcpyAsync((char*)j->h + j->off_rgb, D + j->off_rgb, j->rgb_bytes,
+                               hipMemcpyDeviceToHost, j->stream) != hipSuccess ||
+                (img->counts && hipMemcpyAsync((char*)j->h + j->off_out, D + j->off_out,
+                                               j->out_bytes, hipMemcpyDeviceToHost,
+                                               j->stream) != hipSuccess)) {
+                rc = MDZG_ERR_DEVICE;
+                goto done;
+            }
+        } else if (hipMemcpyAsync((char*)j->h + j->off_out, D + j->off_out, j->out_bytes,
+                                  hipMemcpyDeviceToHost, j->stream) != hipSuccess) {
             rc = MDZG_ERR_DEVICE;
             goto done;
         }
@@ -536,7 +578,7 @@ done:
 
 int mdzg_submit(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, mdzg_job** out)
 {
-    int rc = mdzg_job_start(c, tiles, ntiles, MDZG_BATCH_PERSISTENT, out);
+    int rc = mdzg_job_start(c, tiles, ntiles, MDZG_BATCH_PERSISTENT, NULL, out);
     if (rc == MDZG_OK)
         (*out)->t_start = mdzg_now_ms();
     return rc;
@@ -580,7 +622,14 @@ int mdzg_wait(mdzg_job* j, mdzg_stats* st)
             return MDZG_ERR_DEVICE;
         /* scatter counts to the callers' buffers */
         double pi = 0.0;
-        for (int i = 0; i < j->ntiles; ++i) {
+        if (j->img) {
+            memcpy(j->img->rgb, (char*)j->h + j->off_rgb, j->rgb_bytes);
+            if (j->img->counts)
+                memcpy(j->img->counts, (char*)j->h + j->off_out,
+                       4 * (size_t)j->tiles[0].nlines * (size_t)j->tiles[0].ncols);
+            pi = (double)*(const unsigned long long*)((char*)j->h + j->off_iters);
+        }
+        for (int i = 0; i < j->ntiles && !j->img; ++i) {
             const mdzg_tile* t = &j->tiles[i];
             const uint32_t* src = (const uint32_t*)((char*)j->h + j->out_off[i]);
             size_t n = (size_t)t->nlines * (size_t)t->ncols;
@@ -625,7 +674,94 @@ int mdzg_render_batch(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int mode,
 {
     double t0 = mdzg_now_ms();
     mdzg_job* j = NULL;
-    int rc = mdzg_job_start(c, tiles, ntiles, mode, &j);
+    int rc = mdzg_job_start(c, tiles, ntiles, mode, NULL, &j);
+    if (rc != MDZG_OK)
+        return rc;
+    j->t_start = t0;
+    rc = mdzg_wait(j, st);
+    mdzg_job_free(j);
+    return rc;
+}
+
+int mdzg_palette_ok(const mdzg_palette* p)
+{
+    return p && p->n >= 2 && p->n <= 256 && p->offset >= 0 && p->colour_scale > 0.0 &&
+           p->colour_scale <= 1e300;
+}
+
+int mdzg_colour(mdzg_ctx* c, const uint32_t* counts, int user_w, int user_h, int aa,
+                const mdzg_palette* pal, uint32_t* rgb, mdzg_stats* stats)
+{
+    const double t0 = mdzg_now_ms();
+    if (!c || !counts || !rgb || user_w < 1 || user_h < 1 || aa < 1 || aa > 64 ||
+        !mdzg_palette_ok(pal))
+        return MDZG_ERR_ARG;
+    const uint64_t ns = (uint64_t)user_w * aa * (uint64_t)user_h * aa;
+    const uint64_t np = (uint64_t)user_w * (uint64_t)user_h;
+    if (ns > 0x7fffffffull || (uint64_t)user_w * aa > 0x7fffffffull)
+        return MDZG_ERR_RANGE;
+    const size_t cb = ALIGN_UP(4 * (size_t)ns, 256), pb = 1024, ob = 4 * (size_t)np;
+    const size_t bytes = cb + pb + ob;
+    void *d = NULL, *h = NULL;
+    hipSetDevice(c->device);
+    int slot = acquire(c, bytes, &d, &h);
+    if (slot < 0) {
+        if (hipMalloc(&d, bytes) != hipSuccess)
+            return MDZG_ERR_NOMEM;
+        if (hipHostMalloc(&h, bytes, hipHostMallocDefault) != hipSuccess) {
+            hipFree(d);
+            return MDZG_ERR_NOMEM;
+        }
+    }
+    memcpy(h, counts, 4 * (size_t)ns);
+    memcpy((char*)h + cb, pal->colours, 4 * (size_t)pal->n);
+    hipStream_t st = pick_stream(c);
+    hipEvent_t e0 = NULL, e1 = NULL;
+    int rc = MDZG_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        hipMemcpyAsync(d, h, cb + pb, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipEventRecord(e0, st) != hipSuccess ||
+        mdzg_dev_launch_colour((const uint32_t*)d, user_w * aa, user_w, user_h, aa,
+                               (const uint32_t*)((char*)d + cb), pal->n, pal->offset,
+                               pal->colour_scale, pal->interpolate, 0u,
+                               (uint32_t*)((char*)d + cb + pb), NULL, st) != 0 ||
+        hipEventRecord(e1, st) != hipSuccess ||
+        hipMemcpyAsync((char*)h + cb + pb, (char*)d + cb + pb, ob, hipMemcpyDeviceToHost,
+                       st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        rc = MDZG_ERR_DEVICE;
+    } else {
+        memcpy(rgb, (char*)h + cb + pb, ob);
+        if (stats) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            stats->kernel_ms = ms;
+            stats->wall_ms = mdzg_now_ms() - t0;
+            stats->pixel_iters = 0.0;
+            stats->launches = 1;
+        }
+    }
+    if (rc != MDZG_OK)
+        hipStreamSynchronize(st);      /* nothing may still use the buffers */
+    if (e0)
+        hipEventDestroy(e0);
+    if (e1)
+        hipEventDestroy(e1);
+    if (slot >= 0) {
+        release(c, slot);
+    } else {
+        hipFree(d);
+        hipHostFree(h);
+    }
+    return rc;
+}
+
+int mdzg_rt_render_image(mdzg_ctx* c, const mdzg_tile* tile, const mdzg_image_req* img,
+                         mdzg_stats* st)
+{
+    double t0 = mdzg_now_ms();
+    mdzg_job* j = NULL;
+    int rc = mdzg_job_start(c, tile, 1, MDZG_BATCH_PERSISTENT, img, &j);
     if (rc != MDZG_OK)
         return rc;
     j->t_start = t0;

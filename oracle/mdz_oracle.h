@@ -17,6 +17,7 @@
  *                                src/fractal.c:260-397 (GMP), coords.c:13-18 (MPFR to mpf)
  *   rect from centre/size      : src/coords.c:204-231, :265-313, src/image_info.c:252-280
  *   line pool                  : src/render_threads.c:342-393 (dynamic line claim under a mutex)
+ *   colouring                  : src/palette.c:406-502, src/render.c:108-157 (mdz_colour.c)
  *
  * The third-party arithmetic is MPFR (headers 4.0.2 under /opt/conda, GMP 6.2.1);
  * MPFR's RNDN results are fixed by its correct-rounding guarantee, so the
@@ -109,6 +110,13 @@ int orc_pixel_c(const orc_frame* f, int line, int ix, char* x, char* y, int cap)
 /* Single RN operations at precision p on string operands (hex-float output),
    for per-operation checks: op 0 add, 1 sub, 2 mul. */
 int  orc_rn_op(int op, long precision, const char* a, const char* b, char* out, int cap);
+
+/* Colours of a frame of counts ((user_w*aa) x (user_h*aa) samples) as MDZ's
+   command line computes them: palette_apply for aa == 1, do_anti_aliasing with
+   get_pixel_colour for aa > 1.  0, -1 on bad arguments, -2 when palette_apply's
+   interpolating path would index outside the palette (v >= 2^31). */
+int orc_colour(const uint32_t* counts, int user_w, int user_h, int aa, const uint32_t* pal,
+               int n, int offset, double sc, int ip, uint32_t* rgb);
 
 #ifdef __cplusplus
 }

@@ -68,6 +68,10 @@ def lib():
         L.orc_rn_op.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_char_p,
                                 ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.orc_rn_op.restype = ctypes.c_int
+        L.orc_colour.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_int, ctypes.c_void_p]
+        L.orc_colour.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -162,3 +166,19 @@ def pixel_c(rect, *, precision, width, height, line, ix, path=PATH_MPFR):
     if lib().orc_pixel_c(ctypes.byref(fr), line, ix, a, b, 4096) != 0:
         raise ValueError("bad pixel")
     return a.value.decode(), b.value.decode()
+
+
+def colour(counts, aa, colours, *, offset=0, colour_scale=1.0, interpolate=False):
+    """rgb_data [h, w] (uint32, r | g << 8 | b << 16) of a [h*aa, w*aa] count
+    array, as MDZ's palette_apply / do_anti_aliasing compute it."""
+    c = np.ascontiguousarray(counts, dtype=np.uint32)
+    rh, rw = c.shape
+    pal = np.zeros(256, dtype=np.uint32)
+    cols = np.asarray(colours, dtype=np.uint32)
+    pal[:len(cols)] = cols
+    out = np.zeros((rh // aa, rw // aa), dtype=np.uint32)
+    rc = lib().orc_colour(c.ctypes.data, rw // aa, rh // aa, aa, pal.ctypes.data, len(cols),
+                          offset, colour_scale, int(interpolate), out.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oracle rejected the colouring ({rc})")
+    return out

@@ -1,0 +1,125 @@
+"""GPU colouring: the colour kernel (mdzg_colour) and whole-frame image jobs
+(mdzg_render_image) against the oracle's restatement of MDZ's palette_apply /
+do_anti_aliasing.  Tolerance: none (uint32 rgb_data must be bit-identical)."""
+import os
+
+import numpy as np
+import pytest
+
+import mdz_amd as M
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+MDZ = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mdz")
+
+
+def _pal(n, seed):
+    rng = np.random.default_rng(seed)
+    return [int(x) for x in rng.integers(0, 1 << 32, n, dtype=np.uint64)]
+
+
+def _counts(shape, seed, hi):
+    rng = np.random.default_rng(seed)
+    c = rng.integers(0, hi, shape, dtype=np.int64).astype(np.uint32)
+    c[rng.random(shape) < 0.2] = 0
+    c.flat[:4] = [1, hi - 1, 0, 2]
+    return c
+
+
+@pytest.mark.parametrize("aa", [1, 2, 3, 4])
+@pytest.mark.parametrize("ip", [False, True])
+@pytest.mark.parametrize("n,offset,sc,hi", [(256, 0, 1.0, 100000),
+                                            (256, 148, 0.02539073500000000108, 1 << 31),
+                                            (37, 12, 0.18554686500000000526, 200000),
+                                            (2, 1, 3.7, 1000), (3, 255, 1e-3, 1 << 31),
+                                            (200, 7, 1048576.0, 1 << 31)])
+def test_colour_kernel(ctx, oracle, aa, ip, n, offset, sc, hi):
+    """Random counts (zeros, up to 2^31 - 1), palettes with bits above blue,
+    every aa; scales whose values pass 2^32 exercise the index wrap."""
+    if ip and aa == 1 and hi * sc + offset >= 2 ** 31:
+        hi = int((2 ** 31 - 1 - offset) / sc)     # MDZ's own path is undefined above
+    pal = _pal(n, n + aa)
+    c = _counts((24 * aa, 40 * aa), aa * 13 + n, hi)
+    p = M.make_palette(pal, offset=offset, colour_scale=sc, interpolate=ip)
+    got = ctx.colour(c, aa, p)
+    want = oracle.colour(c, aa, pal, offset=offset, colour_scale=sc, interpolate=ip)
+    assert np.array_equal(got, want), (aa, ip, n, offset, sc)
+
+
+def test_colour_kernel_odd_sizes(ctx, oracle):
+    """Frames that are not a whole number of blocks of 256 pixels."""
+    pal = _pal(256, 5)
+    for (w, h, aa) in [(1, 1, 1), (1, 1, 5), (17, 3, 2), (257, 1, 1), (3, 255, 3)]:
+        c = _counts((h * aa, w * aa), w * h, 50000)
+        p = M.make_palette(pal, offset=5, colour_scale=0.37, interpolate=True)
+        assert np.array_equal(ctx.colour(c, aa, p),
+                              oracle.colour(c, aa, pal, offset=5, colour_scale=0.37,
+                                            interpolate=True)), (w, h, aa)
+
+
+@pytest.mark.parametrize("name,aa", [("floral", 1), ("honeytrace", 2), ("test", 1),
+                                     ("satellite", 2), ("space_pad", 1)])
+def test_render_image_gallery(ctx, oracle, name, aa):
+    """A gallery file rendered and coloured on the device in one job, as
+    `mdz -s file -w 96 -a aa` computes it: equal to the oracle's colouring of
+    the counts, with the counts and the pixel-iteration count as a plain
+    render gives them."""
+    f = os.path.join(MDZ, name + ".mdz")
+    v = M.scene_view(f, 96, aa=aa)
+    v.depth = min(v.depth, 4000)
+    pal = M.scene_palette(f)
+    rgb, cnt, st = ctx.render_image(v, aa, pal, counts=True)
+    ref_counts, st2 = ctx.render(v)
+    assert np.array_equal(cnt, ref_counts)
+    assert st["pixel_iters"] == st2["pixel_iters"] == float(
+        np.where(ref_counts > 0, ref_counts, v.depth).sum())
+    want = oracle.colour(ref_counts, aa, M.palette_colours(pal), offset=pal.offset,
+                         colour_scale=pal.colour_scale, interpolate=bool(pal.interpolate))
+    assert np.array_equal(rgb, want), name
+    rgb2, st3 = ctx.render_image(v, aa, pal)       # counts left in device memory
+    assert np.array_equal(rgb2, rgb) and st3["pixel_iters"] == st["pixel_iters"]
+
+
+def test_render_image_to_png(ctx, tmp_path):
+    from tests.test_colour_cpu import png_decode, _rgb_bytes
+    f = os.path.join(MDZ, "mdz_banner.mdz")
+    v = M.scene_view(f, 120, aa=2)
+    pal = M.scene_palette(f)
+    rgb, _ = ctx.render_image(v, 2, pal)
+    out = tmp_path / "banner.png"
+    M.png_write(str(out), rgb)
+    w, h, img = png_decode(str(out))
+    assert (w, h) == (rgb.shape[1], rgb.shape[0])
+    assert np.array_equal(img, _rgb_bytes(rgb))
+
+
+def test_render_image_fp64_and_julia(ctx, oracle):
+    from mdz_amd import workloads as WL
+    v = WL.config1_view()
+    pal = M.make_palette(_pal(64, 9), offset=3, colour_scale=0.5, interpolate=True)
+    rgb, cnt, st = ctx.render_image(v, 1, pal, counts=True)
+    assert np.array_equal(rgb, oracle.colour(cnt, 1, _pal(64, 9), offset=3, colour_scale=0.5,
+                                             interpolate=True))
+    J = cases.JULIA
+    vj = M.make_view(J["rect"], precision=J["precision"], depth=J["depth"], width=128, height=96,
+                     family=M.FAMILY_JULIA, julia=J["c"])
+    rgb, cnt, _ = ctx.render_image(vj, 2, pal, counts=True)
+    ref, _ = ctx.render(vj)
+    assert np.array_equal(cnt, ref)
+    assert np.array_equal(rgb, oracle.colour(ref, 2, _pal(64, 9), offset=3, colour_scale=0.5,
+                                             interpolate=True))
+
+
+def test_render_image_rejects_bad_arguments(ctx):
+    from mdz_amd import workloads as WL
+    v = WL.config1_view()                          # 240 x 180
+    good = M.make_palette([1, 2, 3])
+    with pytest.raises(M.MdzgError):
+        ctx.render_image(v, 7, good)               # not a whole number of 7x7 blocks
+    bad = M.make_palette([1, 2, 3])
+    bad.n = 1
+    with pytest.raises(M.MdzgError):
+        ctx.render_image(v, 1, bad)
+    with pytest.raises(M.MdzgError):
+        ctx.colour(np.zeros((4, 4), dtype=np.uint32), 2, bad)
