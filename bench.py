@@ -31,6 +31,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "G pixel-iter/s at FP64/128/1024-bit vs MPFR CPU; p50/p99 tile latency"
 PEAK_TOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of k_colour on the 3840x2160 colouring workload by aa:
+# rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 (scaled by a 256 MiB copy's readings)
+COLOUR_TRAFFIC_BYTES = {1: 66.4e6, 2: 166.0e6}
 # HBM bytes per config 2 launch of k_rn_persist<4,true>: rocprofv3 FETCH_SIZE x2 +
 # WRITE_SIZE x1 (both scaled by a 256 MiB copy's readings); the I/O of the
 # kernel does not depend on its arithmetic, so it is measured once
@@ -140,6 +144,51 @@ def gmp_path(ctx, M, WL, a, cpu=True):
     return r
 
 
+def colour_pass(ctx, M, np, a, cpu=True):
+    """The colouring pass (k_colour: palette_apply / do_anti_aliasing) on a
+    3840x2160 image, one sample per pixel and 2x2 anti-aliasing, synthetic
+    counts (0..20000, a tenth never escaped), a 256-colour interpolated
+    palette.  HBM-bound: 4 aa^2 bytes read + 4 written per output pixel."""
+    W, H = 3840, 2160
+    rng = np.random.default_rng(7)
+    pal = M.make_palette([int(x) for x in rng.integers(0, 1 << 24, 256)], offset=148,
+                         colour_scale=0.02539073500000000108, interpolate=True)
+    out = {}
+    steps = max(3, min(a.steps, 20))
+    for aa in (1, 2):
+        c = rng.integers(1, 20000, (H * aa, W * aa), dtype=np.int64).astype(np.uint32)
+        c[rng.random(c.shape) < 0.1] = 0
+        for _ in range(2):
+            ctx.colour(c, aa, pal)
+        kms, walls = [], []
+        for _ in range(steps):
+            _, st = ctx.colour(c, aa, pal, stats=True)
+            kms.append(st["kernel_ms"])
+            walls.append(st["wall_ms"])
+        kavg = sum(kms) / len(kms)
+        nbytes = 4.0 * aa * aa * W * H + 4.0 * W * H
+        gbs = nbytes / (kavg * 1e-3) / 1e9
+        out[f"aa{aa}"] = dict(kernel_ms=kavg, wall_ms_p50=sorted(walls)[len(walls) // 2],
+                              mpix_per_s=W * H / (kavg * 1e-3) / 1e6,
+                              roofline=dict(bound="hbm", achieved=gbs, peak=PEAK_HBM_GBS,
+                                            unit="GB/s", frac=gbs / PEAK_HBM_GBS,
+                                            traffic=COLOUR_TRAFFIC_BYTES[aa],
+                                            bytes_per_launch=nbytes))
+        log(f"colour aa={aa}: {kavg:.3f} ms, {gbs:.0f} GB/s")
+        if cpu and aa == 2:
+            from oracle import oracle as O
+            rows = 64
+            t0 = time.perf_counter()
+            O.colour(c[:rows * aa], aa, M.palette_colours(pal), offset=148,
+                     colour_scale=0.02539073500000000108, interpolate=True)
+            el = time.perf_counter() - t0
+            out["cpu_baseline"] = dict(value=rows * W / el / 1e6, unit="M output pixels/s",
+                                       cores=1, kind="port",
+                                       sample=f"{rows} rows x {W} pixels of the aa=2 image, "
+                                              "MDZ's colouring restated in C (oracle)")
+    return out
+
+
 def crop_tiles(WL, n, seed=0x5E):
     """64x64 crops of the config 2 view at seeded positions (p=128, depth 5000)."""
     import random
@@ -149,9 +198,10 @@ def crop_tiles(WL, n, seed=0x5E):
             for _ in range(n)], v
 
 
-def serve_run(ctx, M, np, mode, tiles, rate, seed=1):
+def serve_run(ctx, M, np, mode, tiles, rate, seed=1, palette=None):
     """Poisson arrivals at `rate` requests/s (inf: all at once); latency per request
-    is done - submit on the host clock; throughput over first submit .. last done."""
+    is done - submit on the host clock; throughput over first submit .. last done.
+    With a palette the requests are image requests (coloured on the device)."""
     import random
     rng = random.Random(seed)
     srv = M.Server(ctx, mode=mode)
@@ -164,7 +214,8 @@ def serve_run(ctx, M, np, mode, tiles, rate, seed=1):
             dt = tnext - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
-        ids.append(srv.submit(v, l, c))
+        ids.append(srv.submit(v, l, c) if palette is None else
+                   srv.submit_image(v, 1, palette, l, c))
     res = [srv.wait(i)[1] for i in ids]
     st = srv.stats()
     srv.close()
@@ -205,6 +256,14 @@ def config4_serving(ctx, M, WL, np, fracs=(0.25, 0.5, 0.75, 0.9), n=300):
             log(f"  load {f:.2f}: {r['tiles_per_s']:.0f} tiles/s p50 {r['p50_ms']:.2f} "
                 f"p99 {r['p99_ms']:.2f} ms")
         out[name] = dict(unloaded_ms=unl, saturation=sat, sweep=sweep)
+    # the same requests as image requests: counts coloured in HBM, colours returned
+    rng = np.random.default_rng(7)
+    pal = M.make_palette([int(x) for x in rng.integers(0, 1 << 24, 256)], offset=148,
+                         colour_scale=0.02539073500000000108, interpolate=True)
+    img = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, float("inf"), palette=pal)
+    log(f"serve batched, image requests: saturation {img['tiles_per_s']:.0f} tiles/s "
+        f"{img['gpix_per_s']:.1f} G pixel-iter/s")
+    out["batched_image_saturation"] = img
     return out
 
 
@@ -411,6 +470,16 @@ def main():
         res.update(config3_batched(ctx, M, WL, np))
     elif a.config == "gmp":
         res.update(gmp_path(ctx, M, WL, a, cpu=not a.no_cpu_baseline))
+    elif a.config == "colour":
+        r = colour_pass(ctx, M, np, a, cpu=not a.no_cpu_baseline)
+        res.update(metric="M output pixels/s of the colouring pass (3840x2160, aa=2)",
+                   unit="M output pixels/s", value=r["aa2"]["mpix_per_s"], steps=a.steps,
+                   warmup=2, ms_per_step=r["aa2"]["kernel_ms"], dtype="f64 colour arithmetic",
+                   config=dict(workload="colouring 3840x2160, 2x2 anti-aliasing, 256-colour "
+                               "interpolated palette", kernel="k_colour"),
+                   roofline=r["aa2"]["roofline"], colour=r)
+        if "cpu_baseline" in r:
+            res["cpu_baseline"] = r["cpu_baseline"]
     elif a.config == "5":
         r = config5_deep(ctx, M, WL, np, cpu=not a.no_cpu_baseline)
         res.update(value=r["gpix_per_s"], steps=1, warmup=0, ms_per_step=1e3 * r["seconds"],

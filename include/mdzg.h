@@ -280,6 +280,12 @@ typedef struct mdzg_serve_stats {
 typedef struct mdzg_server mdzg_server;
 int  mdzg_server_create(mdzg_ctx* ctx, const mdzg_server_cfg* cfg, mdzg_server** out);
 int  mdzg_server_submit(mdzg_server* srv, const mdzg_tile* tile, uint64_t* id);
+/* An image request: the tile (nlines and ncols multiples of aa) is rendered
+   like any request and, when done, coloured on the device as mdzg_colour
+   colours (the palette is copied).  rgb receives (ncols/aa) x (nlines/aa)
+   colours; tile->counts receives the counts unless it is NULL. */
+int  mdzg_server_submit_image(mdzg_server* srv, const mdzg_tile* tile, int aa,
+                              const mdzg_palette* pal, uint32_t* rgb, uint64_t* id);
 int  mdzg_server_poll(mdzg_server* srv, uint64_t id);     /* 1 done, 0 not yet */
 int  mdzg_server_wait(mdzg_server* srv, uint64_t id, mdzg_req_info* info);
 int  mdzg_server_info(mdzg_server* srv, uint64_t id, mdzg_req_info* info);

@@ -140,6 +140,8 @@ def lib():
                                         P(ctypes.c_uint32), P(ctypes.c_uint32), P(Stats)]
         L.mdzg_png_write.argtypes = [ctypes.c_char_p, P(ctypes.c_uint32), ctypes.c_int,
                                      ctypes.c_int]
+        L.mdzg_server_submit_image.argtypes = [vp, P(Tile), ctypes.c_int, P(Palette),
+                                               P(ctypes.c_uint32), P(ctypes.c_uint64)]
         L.mdzg_scene_rect.argtypes = [P(Scene), ctypes.c_int, ctypes.c_int, ctypes.c_int] + \
             [ctypes.c_char_p] * 4 + [ctypes.c_int]
         L.mdzg_render_tile.argtypes = [vp, P(Tile), P(Stats)]
@@ -463,6 +465,23 @@ class Server:
         self._live[rid.value] = (t, out, view)
         return rid.value
 
+    def submit_image(self, view, aa, palette, lines=None, cols=None, counts=False):
+        """An image request (mdzg_server_submit_image): wait(id) returns
+        (rgb, info), or ((rgb, counts), info) with counts=True."""
+        t, out = Context._tile(view, lines, cols)
+        if t.nlines % aa or t.ncols % aa:
+            raise ValueError("tile is not a whole number of aa x aa blocks")
+        rgb = np.zeros((t.nlines // aa, t.ncols // aa), dtype=np.uint32)
+        if not counts:
+            t.counts = ctypes.POINTER(ctypes.c_uint32)()
+        pal = Palette.from_buffer_copy(palette)
+        rid = ctypes.c_uint64()
+        _check(lib().mdzg_server_submit_image(self._s, ctypes.byref(t), aa, ctypes.byref(pal),
+                                              rgb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                              ctypes.byref(rid)), "submit_image")
+        self._live[rid.value] = (t, ((rgb, out) if counts else rgb), view, pal)
+        return rid.value
+
     def poll(self, rid):
         return _check(lib().mdzg_server_poll(self._s, rid), "poll") == 1
 
@@ -470,7 +489,7 @@ class Server:
         info = ReqInfo()
         rc = lib().mdzg_server_wait(self._s, rid, ctypes.byref(info))
         _check(rc, "wait")
-        t, out, view = self._live.pop(rid)
+        out = self._live.pop(rid)[1]
         return out, info.as_dict()
 
     def info(self, rid):

@@ -21,11 +21,21 @@
  * the frame (count, or depth for a pixel that never escaped) into a 64-bit
  * device counter: the escape-time counts need not leave HBM.
  *
- * HBM-bound: 4 aa^2 bytes read and 4 bytes written per output pixel; the
- * palette (<= 256 colours) is staged in LDS.
+ * HBM-bound: 4 aa^2 bytes read and 4 bytes written per output pixel.  The
+ * grid is sized to what stays resident and strides over the image, so each
+ * block stages the palette (<= 256 colours) in LDS once; aa = 1 moves 4
+ * pixels per lane and aa = 2 two output pixels per lane with 16-byte loads.
+ * Palette indexes are reduced modulo n - 1 by Lemire's multiply-based
+ * remainder (Lemire, Kaser, Kurz, "Faster remainder by direct computation",
+ * 2019), exact for every 32-bit index.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "colour_math.h"
+
+using mdzg::colour::colour_apply;
+using mdzg::colour::colour_get;
 
 struct colour_args {
     const uint32_t* counts;      /* real_w x real_h, row-major                  */
@@ -33,6 +43,7 @@ struct colour_args {
     uint32_t* rgb;               /* user_w x user_h                             */
     unsigned long long* iters;   /* pixel-iteration sum (added to), may be null */
     double sc;                   /* image_info.colour_scale                     */
+    uint64_t mmagic;             /* remainder constant for n - 1                */
     int real_w, user_w, user_h, aa;
     int n;                       /* pal_indexes, 2..256                         */
     int offset;                  /* pal_offset                                  */
@@ -40,55 +51,44 @@ struct colour_args {
     uint32_t depth;
 };
 
-__device__ __forceinline__ uint32_t u32_of(double v)
+struct pal_ctx {
+    mdzg::colour::palette_view P;
+    double sc;
+    int offset;
+};
+
+/* IP: interpolating palette; COUNT: sum the pixel-iterations */
+template <bool IP, bool COUNT>
+__device__ __forceinline__ uint32_t apply1(const pal_ctx& p, uint32_t raw, uint64_t& it,
+                                           uint32_t depth)
 {
-    return v < 9223372036854775808.0 ? (uint32_t)(uint64_t)(int64_t)v : 0u;
+    if (COUNT)
+        it += raw ? raw : depth;
+    return colour_apply<IP>(p.P, raw, p.sc, p.offset);
 }
 
-__device__ __forceinline__ uint32_t u8_of(double v)
+/* one sample of the anti-alias box, its channels added to r, g, b */
+template <bool IP, bool COUNT>
+__device__ __forceinline__ void sample(const pal_ctx& p, uint32_t raw, uint32_t& r, uint32_t& g,
+                                       uint32_t& b, uint64_t& it, uint32_t depth)
 {
-    return (uint32_t)v & 0xffu;
+    if (COUNT)
+        it += raw ? raw : depth;
+    const uint32_t c = colour_get<IP>(p.P, raw, p.sc, p.offset);
+    r += c & 0xffu;
+    g += (c >> 8) & 0xffu;
+    b += (c >> 16) & 0xffu;
 }
 
-/* one interpolated colour: the shared body of both interpolating paths */
-__device__ __forceinline__ uint32_t blend(const uint32_t* pal, uint32_t m, double v, double cval)
+__device__ __forceinline__ uint32_t pack_avg(uint32_t r, uint32_t g, uint32_t b, uint32_t aaaa)
 {
-    const double diff = cval - v;
-    const double rdiff = 1.0 - diff;
-    const uint32_t c1 = pal[u32_of(floor(v)) % m];
-    const uint32_t c2 = pal[u32_of(cval) % m];
-    const uint32_t r = u8_of(diff * (double)(c1 & 0xffu) + rdiff * (double)(c2 & 0xffu));
-    const uint32_t g = u8_of(diff * (double)((c1 >> 8) & 0xffu) + rdiff * (double)((c2 >> 8) & 0xffu));
-    const uint32_t b = u8_of(diff * (double)((c1 >> 16) & 0xffu) + rdiff * (double)((c2 >> 16) & 0xffu));
-    return r | (g << 8) | (b << 16);
+    return (r / aaaa) | ((g / aaaa) << 8) | ((b / aaaa) << 16);
 }
 
-/* palette_apply, one sample */
-__device__ __forceinline__ uint32_t colour_apply(const uint32_t* pal, uint32_t m, uint32_t raw,
-                                                 double sc, int offset, int ip)
-{
-    if (!raw)
-        return 0u;
-    const double v = (double)(int32_t)raw * sc + (double)offset;
-    if (ip)
-        return blend(pal, m, v, ceil(v));
-    return pal[u32_of(v) % m];
-}
-
-/* get_pixel_colour(raw * sc, ip) */
-__device__ __forceinline__ uint32_t colour_get(const uint32_t* pal, uint32_t m, uint32_t raw,
-                                               double sc, int offset, int ip)
-{
-    double v = (double)(int32_t)raw * sc;
-    if (v != 0.0)
-        v += (double)offset;
-    if (ip)
-        return blend(pal, m, v, ceil(v));
-    if (v == 0.0)
-        return 0u;
-    return pal[u32_of(v) % m];
-}
-
+/* FORM 1: aa = 1, counts contiguous (real_w = user_w), 4 pixels per lane;
+   FORM 2: aa = 2, user_w even, 2 output pixels (4 x 2 samples) per lane;
+   FORM 0: any aa, one output pixel per lane */
+template <int FORM, bool IP, bool COUNT>
 __global__ __launch_bounds__(256) void k_colour(colour_args a)
 {
     __shared__ uint32_t lpal[256];
@@ -97,42 +97,85 @@ __global__ __launch_bounds__(256) void k_colour(colour_args a)
         lpal[i] = a.palette[i];
     __syncthreads();
 
+    pal_ctx p;
+    p.P.pal = lpal;
+    p.P.M = a.mmagic;
+    p.P.m = (uint32_t)(a.n - 1);
+    p.sc = a.sc;
+    p.offset = a.offset;
+    const uint32_t depth = a.depth;
     const uint32_t npix = (uint32_t)a.user_w * (uint32_t)a.user_h;
-    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t m = (uint32_t)(a.n - 1);
-    unsigned long long it = 0;
-    if (idx < npix) {
-        const uint32_t y = idx / (uint32_t)a.user_w;
-        const uint32_t x = idx - y * (uint32_t)a.user_w;
-        const int aa = a.aa;
-        uint32_t out;
-        if (aa == 1) {
-            const uint32_t raw = a.counts[(size_t)y * (size_t)a.real_w + x];
-            it = raw ? raw : a.depth;
-            out = colour_apply(lpal, m, raw, a.sc, a.offset, a.ip);
-        } else {
-            uint32_t r = 0, g = 0, b = 0;
-            for (int yi = 0; yi < aa; ++yi) {
-                const uint32_t* row = a.counts + ((size_t)y * aa + yi) * (size_t)a.real_w +
-                                      (size_t)x * aa;
-                for (int xi = 0; xi < aa; ++xi) {
-                    const uint32_t raw = row[xi];
-                    it += raw ? raw : a.depth;
-                    const uint32_t c = colour_get(lpal, m, raw, a.sc, a.offset, a.ip);
-                    r += c & 0xffu;
-                    g += (c >> 8) & 0xffu;
-                    b += (c >> 16) & 0xffu;
-                }
-            }
-            const uint32_t aaaa = (uint32_t)(aa * aa);
-            out = (r / aaaa) | ((g / aaaa) << 8) | ((b / aaaa) << 16);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t it = 0;
+
+    if (FORM == 1) {
+        const uint32_t nq = npix >> 2;
+        const uint4* src = (const uint4*)a.counts;
+        uint4* dst = (uint4*)a.rgb;
+        for (uint32_t q = t0; q < nq; q += stride) {
+            const uint4 c = src[q];
+            uint4 o;
+            o.x = apply1<IP, COUNT>(p, c.x, it, depth);
+            o.y = apply1<IP, COUNT>(p, c.y, it, depth);
+            o.z = apply1<IP, COUNT>(p, c.z, it, depth);
+            o.w = apply1<IP, COUNT>(p, c.w, it, depth);
+            dst[q] = o;
         }
-        a.rgb[idx] = out;
+        const uint32_t k = (nq << 2) + t0;           /* the last npix % 4 pixels */
+        if (t0 < (npix & 3u))
+            a.rgb[k] = apply1<IP, COUNT>(p, a.counts[k], it, depth);
+    } else if (FORM == 2) {
+        const uint32_t hw = (uint32_t)a.user_w >> 1;
+        const uint32_t npair = npix >> 1;
+        for (uint32_t q = t0; q < npair; q += stride) {
+            const uint32_t y = q / hw;
+            const uint32_t x = (q - y * hw) << 1;
+            const uint32_t* r0 = a.counts + (size_t)(2 * y) * (size_t)a.real_w + 2 * x;
+            const uint4 u = *(const uint4*)r0;
+            const uint4 v = *(const uint4*)(r0 + a.real_w);
+            uint32_t r = 0, g = 0, b = 0;
+            sample<IP, COUNT>(p, u.x, r, g, b, it, depth);
+            sample<IP, COUNT>(p, u.y, r, g, b, it, depth);
+            sample<IP, COUNT>(p, v.x, r, g, b, it, depth);
+            sample<IP, COUNT>(p, v.y, r, g, b, it, depth);
+            uint2 o;
+            o.x = pack_avg(r, g, b, 4u);
+            r = g = b = 0;
+            sample<IP, COUNT>(p, u.z, r, g, b, it, depth);
+            sample<IP, COUNT>(p, u.w, r, g, b, it, depth);
+            sample<IP, COUNT>(p, v.z, r, g, b, it, depth);
+            sample<IP, COUNT>(p, v.w, r, g, b, it, depth);
+            o.y = pack_avg(r, g, b, 4u);
+            *(uint2*)(a.rgb + (size_t)y * (size_t)a.user_w + x) = o;
+        }
+    } else {
+        const int aa = a.aa;
+        const uint32_t aaaa = (uint32_t)(aa * aa);
+        for (uint32_t q = t0; q < npix; q += stride) {
+            const uint32_t y = q / (uint32_t)a.user_w;
+            const uint32_t x = q - y * (uint32_t)a.user_w;
+            uint32_t out;
+            if (aa == 1) {
+                out = apply1<IP, COUNT>(p, a.counts[(size_t)y * (size_t)a.real_w + x], it, depth);
+            } else {
+                uint32_t r = 0, g = 0, b = 0;
+                for (int yi = 0; yi < aa; ++yi) {
+                    const uint32_t* row = a.counts + ((size_t)y * aa + yi) * (size_t)a.real_w +
+                                          (size_t)x * aa;
+                    for (int xi = 0; xi < aa; ++xi)
+                        sample<IP, COUNT>(p, row[xi], r, g, b, it, depth);
+                }
+                out = pack_avg(r, g, b, aaaa);
+            }
+            a.rgb[q] = out;
+        }
     }
-    if (a.iters) {
+
+    if (COUNT) {
         /* wave sum, then one atomic per block */
         for (int o = 32; o > 0; o >>= 1)
-            it += __shfl_xor(it, o, 64);
+            it += __shfl_xor((unsigned long long)it, o, 64);
         const int wv = threadIdx.x >> 6;
         if ((threadIdx.x & 63) == 0)
             lsum[wv] = it;
@@ -146,13 +189,16 @@ __global__ __launch_bounds__(256) void k_colour(colour_args a)
     }
 }
 
+typedef void (*kfn)(colour_args);
+
 extern "C" int mdzg_dev_launch_colour(const uint32_t* counts, int real_w, int user_w, int user_h,
                                       int aa, const uint32_t* palette, int n, int offset,
                                       double sc, int ip, uint32_t depth, uint32_t* rgb,
-                                      unsigned long long* iters, hipStream_t st)
+                                      unsigned long long* iters, int max_blocks, hipStream_t st)
 {
     if (!counts || !palette || !rgb || user_w < 1 || user_h < 1 || aa < 1 || n < 2 || n > 256 ||
-        real_w < user_w * aa || (uint64_t)user_w * (uint64_t)user_h > 0x7fffffffull)
+        real_w < user_w * aa || (uint64_t)user_w * (uint64_t)user_h > 0x7fffffffull ||
+        max_blocks < 1)
         return -1;
     colour_args a;
     a.counts = counts;
@@ -160,6 +206,7 @@ extern "C" int mdzg_dev_launch_colour(const uint32_t* counts, int real_w, int us
     a.rgb = rgb;
     a.iters = iters;
     a.sc = sc;
+    a.mmagic = mdzg::colour::rem_magic((uint32_t)(n - 1));
     a.real_w = real_w;
     a.user_w = user_w;
     a.user_h = user_h;
@@ -169,6 +216,29 @@ extern "C" int mdzg_dev_launch_colour(const uint32_t* counts, int real_w, int us
     a.ip = ip;
     a.depth = depth;
     const uint32_t npix = (uint32_t)user_w * (uint32_t)user_h;
-    hipLaunchKernelGGL(k_colour, dim3((npix + 255) / 256), dim3(256), 0, st, a);
+    const int aligned = ((uintptr_t)counts % 16 == 0) && ((uintptr_t)rgb % 16 == 0);
+    int form = 0;
+    uint32_t work = npix;
+    if (aligned && aa == 1 && real_w == user_w) {
+        form = 1;
+        work = (npix + 3) / 4;
+    } else if (aligned && aa == 2 && user_w % 2 == 0 && real_w == 2 * user_w) {
+        form = 2;
+        work = npix / 2;
+    }
+    uint32_t grid = (work + 255) / 256;
+    if (grid > (uint32_t)max_blocks)
+        grid = (uint32_t)max_blocks;
+    if (grid < 1)
+        grid = 1;
+    static const kfn table[3][2][2] = {
+        { { k_colour<0, false, false>, k_colour<0, false, true> },
+          { k_colour<0, true, false>, k_colour<0, true, true> } },
+        { { k_colour<1, false, false>, k_colour<1, false, true> },
+          { k_colour<1, true, false>, k_colour<1, true, true> } },
+        { { k_colour<2, false, false>, k_colour<2, false, true> },
+          { k_colour<2, true, false>, k_colour<2, true, true> } },
+    };
+    hipLaunchKernelGGL(table[form][ip ? 1 : 0][iters ? 1 : 0], dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -27,14 +27,15 @@ int mdzg_dev_launch_slice(int kid, const mdzg_dtile* d_tiles, int ntiles, uint64
                           unsigned long long* d_counter, int grid, int budget, hipStream_t st);
 
 /* k_colour.hip: counts -> RGB (palette_apply / do_anti_aliasing), adding the
-   pixel-iterations of the counts read to *iters when iters is not null */
+   pixel-iterations of the counts read to *iters when iters is not null; at
+   most max_blocks blocks of 256 lanes stride over the image */
 int mdzg_dev_launch_colour(const uint32_t* counts, int real_w, int user_w, int user_h, int aa,
                            const uint32_t* palette, int n, int offset, double sc, int ip,
                            uint32_t depth, uint32_t* rgb, unsigned long long* iters,
-                           hipStream_t st);
+                           int max_blocks, hipStream_t st);
 
-/* An image job: the one tile of the job is a whole frame of user_w*aa x
-   user_h*aa samples, coloured on the device after its escape-time kernels. */
+/* An image job: the one tile of the job, user_w*aa x user_h*aa samples,
+   coloured on the device after its escape-time kernels. */
 typedef struct mdzg_image_req {
     int user_w, user_h, aa;
     const mdzg_palette* pal;
@@ -43,6 +44,12 @@ typedef struct mdzg_image_req {
 } mdzg_image_req;
 int mdzg_rt_render_image(mdzg_ctx* c, const mdzg_tile* tile, const mdzg_image_req* img,
                          mdzg_stats* st);
+/* the same as an asynchronous job (mdzg_wait / mdzg_job_free); img must stay
+   valid until the job is waited for */
+int mdzg_rt_submit_image(mdzg_ctx* c, const mdzg_tile* tile, const mdzg_image_req* img,
+                         mdzg_job** out);
+/* resident blocks per CU of the colouring pass's grid */
+#define MDZG_COLOUR_BLOCKS_PER_CU 8
 /* validation shared by the colouring entry points */
 int mdzg_palette_ok(const mdzg_palette* p);
 

@@ -535,7 +535,8 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
                                        (const uint32_t*)(D + j->off_pal), P->n, P->offset,
                                        P->colour_scale, P->interpolate,
                                        (uint32_t)tiles[0].view.depth, (uint32_t*)(D + j->off_rgb),
-                                       (unsigned long long*)(D + j->off_iters), j->stream) != 0) {
+                                       (unsigned long long*)(D + j->off_iters),
+                                       MDZG_COLOUR_BLOCKS_PER_CU * c->ncu, j->stream) != 0) {
                 rc = MDZG_ERR_DEVICE;
                 goto done;
             }
@@ -579,6 +580,15 @@ done:
 int mdzg_submit(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, mdzg_job** out)
 {
     int rc = mdzg_job_start(c, tiles, ntiles, MDZG_BATCH_PERSISTENT, NULL, out);
+    if (rc == MDZG_OK)
+        (*out)->t_start = mdzg_now_ms();
+    return rc;
+}
+
+int mdzg_rt_submit_image(mdzg_ctx* c, const mdzg_tile* tile, const mdzg_image_req* img,
+                         mdzg_job** out)
+{
+    int rc = mdzg_job_start(c, tile, 1, MDZG_BATCH_PERSISTENT, img, out);
     if (rc == MDZG_OK)
         (*out)->t_start = mdzg_now_ms();
     return rc;
@@ -724,7 +734,8 @@ int mdzg_colour(mdzg_ctx* c, const uint32_t* counts, int user_w, int user_h, int
         mdzg_dev_launch_colour((const uint32_t*)d, user_w * aa, user_w, user_h, aa,
                                (const uint32_t*)((char*)d + cb), pal->n, pal->offset,
                                pal->colour_scale, pal->interpolate, 0u,
-                               (uint32_t*)((char*)d + cb + pb), NULL, st) != 0 ||
+                               (uint32_t*)((char*)d + cb + pb), NULL,
+                               MDZG_COLOUR_BLOCKS_PER_CU * c->ncu, st) != 0 ||
         hipEventRecord(e1, st) != hipSuccess ||
         hipMemcpyAsync((char*)h + cb + pb, (char*)d + cb + pb, ob, hipMemcpyDeviceToHost,
                        st) != hipSuccess ||

@@ -55,6 +55,13 @@ typedef struct sreq {
     uint32_t *d_xs, *d_ys, *d_jc, *d_out, *d_state;
     int32_t* d_wzs;
     double xmin, ymax, width, jre, jim;
+    /* image requests: counts coloured on the device when the request is done */
+    int image;
+    int want_counts;
+    mdzg_palette pal;
+    mdzg_image_req ir;          /* per-request mode */
+    uint32_t *d_rgb, *d_pal;
+    unsigned long long* d_iters;
 } sreq;
 
 typedef struct ablock {
@@ -177,6 +184,8 @@ static size_t req_bytes(const sreq* r)
     }
     b += ALIGN_UP(npix * 4, 256);                        /* counts */
     b += ALIGN_UP(npix * 4, 256);                        /* iterations done */
+    if (r->image)                                        /* colours, palette, iteration sum */
+        b += ALIGN_UP((size_t)r->ir.user_w * (size_t)r->ir.user_h * 4, 256) + 1024 + 256;
     return b;
 }
 
@@ -213,10 +222,13 @@ static int stage_new(mdzg_server* s, sreq** nw, int nnew, hipStream_t st)
 {
     /* start values go through the pinned staging buffer */
     size_t need = 0;
-    for (int i = 0; i < nnew; ++i)
+    for (int i = 0; i < nnew; ++i) {
         if (nw[i]->pv.vs)
             need += ALIGN_UP(((size_t)nw[i]->tile.ncols + (size_t)nw[i]->tile.nlines + 2) *
                              4 * (size_t)nw[i]->pv.vs, 256);
+        if (nw[i]->image)
+            need += 1024;
+    }
     if (need > s->capstage) {
         if (s->h_stage) {
             hipStreamSynchronize(st);
@@ -262,6 +274,20 @@ static int stage_new(mdzg_server* s, sreq** nw, int nnew, hipStream_t st)
         r->d_out = (uint32_t*)(base + o);
         o += ALIGN_UP((size_t)r->npix * 4, 256);
         r->d_wzs = (int32_t*)(base + o);
+        o += ALIGN_UP((size_t)r->npix * 4, 256);
+        if (r->image) {
+            r->d_rgb = (uint32_t*)(base + o);
+            o += ALIGN_UP((size_t)r->ir.user_w * (size_t)r->ir.user_h * 4, 256);
+            r->d_pal = (uint32_t*)(base + o);
+            r->d_iters = (unsigned long long*)(base + o + 1024);
+            char* H = (char*)s->h_stage + hoff;
+            memcpy(H, r->pal.colours, 4 * (size_t)r->pal.n);
+            if (hipMemcpyAsync(r->d_pal, H, 4 * (size_t)r->pal.n, hipMemcpyHostToDevice, st) !=
+                    hipSuccess ||
+                hipMemsetD32Async((hipDeviceptr_t)r->d_iters, 0, 2, st) != hipSuccess)
+                return MDZG_ERR_DEVICE;
+            hoff += 1024;
+        }
         if (hipMemsetD32Async((hipDeviceptr_t)r->d_out, (int)SENT, r->npix, st) != hipSuccess ||
             hipMemsetD32Async((hipDeviceptr_t)r->d_wzs, 0, r->npix, st) != hipSuccess ||
             hipMemsetD32Async((hipDeviceptr_t)(s->d_remaining + r->slot), (int)r->npix, 1, st) !=
@@ -362,6 +388,37 @@ static int run_slice(mdzg_server* s)
     return MDZG_OK;
 }
 
+/* a finished request's counts to the client */
+static int retire_counts(sreq* r)
+{
+    if (hipMemcpy(r->tile.counts, r->d_out, (size_t)r->npix * 4, hipMemcpyDeviceToHost) !=
+        hipSuccess)
+        return MDZG_ERR_DEVICE;
+    r->pixel_iters = count_iters(r->tile.counts, r->npix, (double)r->tile.view.depth);
+    return MDZG_OK;
+}
+
+/* a finished image request: its counts coloured where they lie, the colours
+   (and the counts, if asked for) to the client */
+static int retire_image(mdzg_server* s, sreq* r)
+{
+    hipStream_t st = s->streams[0];
+    unsigned long long iters = 0;
+    const size_t nrgb = (size_t)r->ir.user_w * (size_t)r->ir.user_h;
+    if (mdzg_dev_launch_colour(r->d_out, r->tile.ncols, r->ir.user_w, r->ir.user_h, r->ir.aa,
+                               r->d_pal, r->pal.n, r->pal.offset, r->pal.colour_scale,
+                               r->pal.interpolate, (uint32_t)r->tile.view.depth, r->d_rgb,
+                               r->d_iters, MDZG_COLOUR_BLOCKS_PER_CU * s->ctx->ncu, st) != 0 ||
+        hipMemcpyAsync(r->ir.rgb, r->d_rgb, nrgb * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&iters, r->d_iters, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        (r->want_counts && hipMemcpyAsync(r->tile.counts, r->d_out, (size_t)r->npix * 4,
+                                          hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return MDZG_ERR_DEVICE;
+    r->pixel_iters = (double)iters;
+    return MDZG_OK;
+}
+
 static void* sched_batched(void* arg)
 {
     mdzg_server* s = (mdzg_server*)arg;
@@ -430,11 +487,10 @@ static void* sched_batched(void* arg)
             }
             if (rc == MDZG_OK) {
                 pthread_mutex_unlock(&s->lock);
-                hipMemcpy(r->tile.counts, r->d_out, (size_t)r->npix * 4, hipMemcpyDeviceToHost);
-                r->pixel_iters = count_iters(r->tile.counts, r->npix, (double)r->tile.view.depth);
+                const int rr = r->image ? retire_image(s, r) : retire_counts(r);
                 pthread_mutex_lock(&s->lock);
-                r->status = 2;
-                s->st.completed++;
+                r->status = rr == MDZG_OK ? 2 : rr;
+                s->st.completed += rr == MDZG_OK;
                 s->st.pixel_iters += r->pixel_iters;
             } else {
                 r->status = rc;
@@ -473,7 +529,8 @@ static void* sched_per_request(void* arg)
         while (s->pend_head != s->pend_tail) {
             sreq* r = s->reqs[s->pend[s->pend_head++]];
             pthread_mutex_unlock(&s->lock);
-            int rc = mdzg_submit(s->ctx, &r->tile, 1, &r->job);
+            int rc = r->image ? mdzg_rt_submit_image(s->ctx, &r->tile, &r->ir, &r->job)
+                              : mdzg_submit(s->ctx, &r->tile, 1, &r->job);
             pthread_mutex_lock(&s->lock);
             r->t_admit = mdzg_now_ms();
             s->st.admitted++;
@@ -607,11 +664,17 @@ fail:
     return rc;
 }
 
-int mdzg_server_submit(mdzg_server* s, const mdzg_tile* t, uint64_t* id)
+static int submit(mdzg_server* s, const mdzg_tile* t, int aa, const mdzg_palette* pal,
+                  uint32_t* rgb, uint64_t* id)
 {
-    if (!s || !t || !id || !t->counts || t->nlines < 1 || t->ncols < 1 || t->line0 < 0 ||
+    if (!s || !t || !id || t->nlines < 1 || t->ncols < 1 || t->line0 < 0 ||
         t->col0 < 0 || t->line0 + t->nlines > t->view.real_height ||
-        t->col0 + t->ncols > t->view.real_width)
+        t->col0 + t->ncols > t->view.real_width ||
+        (uint64_t)t->nlines * (uint64_t)t->ncols > 0x7fffffffull)
+        return MDZG_ERR_ARG;
+    if (pal ? (!rgb || aa < 1 || aa > 64 || !mdzg_palette_ok(pal) || t->nlines % aa ||
+               t->ncols % aa)
+            : !t->counts)
         return MDZG_ERR_ARG;
     sreq* r = (sreq*)calloc(1, sizeof *r);
     if (!r)
@@ -619,6 +682,19 @@ int mdzg_server_submit(mdzg_server* s, const mdzg_tile* t, uint64_t* id)
     r->tile = *t;
     r->npix = (uint32_t)t->nlines * (uint32_t)t->ncols;
     r->slot = -1;
+    if (pal) {
+        r->image = 1;
+        r->want_counts = t->counts != NULL;
+        r->pal = *pal;
+        r->ir.user_w = t->ncols / aa;
+        r->ir.user_h = t->nlines / aa;
+        r->ir.aa = aa;
+        r->ir.pal = &r->pal;
+        r->ir.rgb = rgb;
+        r->ir.counts = t->counts;
+        if (!r->tile.counts)
+            r->tile.counts = rgb;   /* image jobs never scatter into it */
+    }
     int rc = MDZG_OK;
     if (s->cfg.mode == MDZG_SERVE_BATCHED) {
         rc = mdzg_view_parse(&t->view, &r->pv);
@@ -659,6 +735,19 @@ int mdzg_server_submit(mdzg_server* s, const mdzg_tile* t, uint64_t* id)
     pthread_cond_signal(&s->cv_work);
     pthread_mutex_unlock(&s->lock);
     return rc;
+}
+
+int mdzg_server_submit(mdzg_server* s, const mdzg_tile* t, uint64_t* id)
+{
+    return submit(s, t, 0, NULL, NULL, id);
+}
+
+int mdzg_server_submit_image(mdzg_server* s, const mdzg_tile* t, int aa, const mdzg_palette* pal,
+                             uint32_t* rgb, uint64_t* id)
+{
+    if (!pal)
+        return MDZG_ERR_ARG;
+    return submit(s, t, aa, pal, rgb, id);
 }
 
 static sreq* find_req(mdzg_server* s, uint64_t id)

@@ -284,3 +284,14 @@ def test_png_bad_path():
     import mdz_amd as M
     with pytest.raises(M.MdzgError):
         M.png_write("/nonexistent-dir/a.png", np.zeros((2, 2), dtype=np.uint32))
+
+
+def test_colour_helpers_on_the_host(tmp_path):
+    """The colour kernel's remainder and conversion helpers, built for the
+    host from the same header, against % and the truncation they restate."""
+    import subprocess
+    exe = tmp_path / "colour_selftest"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-o", str(exe),
+                    os.path.join(ROOT, "tests", "colour_selftest.cpp")], check=True)
+    r = subprocess.run([str(exe), "100000"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr

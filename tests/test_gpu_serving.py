@@ -103,3 +103,54 @@ def test_server_arrivals_while_running(ctx):
     for k, (o, info) in enumerate(outs):
         assert _eq(o, ref[8 * k:8 * k + 8, :64])
         assert info["admit_ms"] < info_big["done_ms"]     # admitted while the big one ran
+
+
+@pytest.mark.parametrize("mode", [M.SERVE_BATCHED, M.SERVE_PER_REQUEST])
+def test_server_image_requests(ctx, oracle, mode):
+    """Image requests mixed with count requests: colours equal the oracle's
+    colouring of the standalone counts (aa 1..3, interpolated or not), counts
+    returned when asked, pixel-iterations as counted from the counts."""
+    reqs = _mix()
+    rng = np.random.default_rng(3)
+    cols = [int(x) for x in rng.integers(0, 1 << 32, 256, dtype=np.uint64)]
+    plans = []
+    for k, (v, (l0, nl), (c0, nc)) in enumerate(reqs):
+        aa = (1, 2, 3)[k % 3]
+        nl, nc = nl - nl % aa, nc - nc % aa
+        ip = bool(k % 2)
+        pal = M.make_palette(cols[:64 + 12 * k], offset=(17 * k) % 256,
+                             colour_scale=(0.37, 1.0, 0.0253907)[k % 3], interpolate=ip)
+        plans.append((v, (l0, nl), (c0, nc), aa, pal, k % 4 == 0))
+    alone = [ctx.render(v, lines=l, cols=c)[0] for v, l, c, _, _, _ in plans]
+    with M.Server(ctx, mode=mode, slice_iters=9 if mode == M.SERVE_BATCHED else 0) as srv:
+        ids = []
+        for k, (v, l, c, aa, pal, plain) in enumerate(plans):
+            if plain:
+                ids.append(srv.submit(v, l, c))
+            else:
+                ids.append(srv.submit_image(v, aa, pal, l, c, counts=(k % 4 == 1)))
+        outs = [srv.wait(i) for i in ids]
+    for k, ((v, l, c, aa, pal, plain), ref, (out, info)) in enumerate(zip(plans, alone, outs)):
+        assert info["status"] == 2
+        assert info["pixel_iters"] == float(np.where(ref > 0, ref, v.depth).sum())
+        if plain:
+            assert _eq(out, ref)
+            continue
+        rgb = out[0] if k % 4 == 1 else out
+        if k % 4 == 1:
+            assert _eq(out[1], ref)
+        want = oracle.colour(ref, aa, M.palette_colours(pal), offset=pal.offset,
+                             colour_scale=pal.colour_scale, interpolate=bool(pal.interpolate))
+        assert np.array_equal(rgb, want), k
+
+
+def test_server_image_request_arguments(ctx):
+    v = WL.config1_view()
+    pal = M.make_palette([1, 2, 3])
+    with M.Server(ctx) as srv:
+        with pytest.raises(ValueError):
+            srv.submit_image(v, 2, pal, (0, 5), (0, 8))      # 5 rows: not whole 2x2 blocks
+        bad = M.make_palette([1, 2, 3])
+        bad.n = 300
+        with pytest.raises(M.MdzgError):
+            srv.submit_image(v, 1, bad, (0, 4), (0, 4))
