@@ -61,7 +61,7 @@ def host_cores():
     except OSError:
         pass
     n = n or (os.cpu_count() or 1)
-    # the GPU box gives one GPU's job a 16-core share
+    # a host that shares its cores between GPUs caps a job's threads in OMP_NUM_THREADS
     cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
     return min(n, cap) if cap > 0 else n
 

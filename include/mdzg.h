@@ -60,6 +60,7 @@ enum { MDZG_FAMILY_MANDEL = 0, MDZG_FAMILY_JULIA = 1 };
 enum { MDZG_PATH_MPFR = 0, MDZG_PATH_LD = 1, MDZG_PATH_DOUBLE = 2, MDZG_PATH_GMP = 3 };
 
 #define MDZG_MAX_PRECISION 1024   /* largest p of the MPFR and GMP paths built for the device */
+#define MDZG_MIN_PRECISION 2      /* smallest p of the MPFR and GMP paths                  */
 
 /* The image_info fields the line workers read (src/image_info.h:63-122). */
 typedef struct mdzg_view {

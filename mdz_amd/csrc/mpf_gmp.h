@@ -25,6 +25,8 @@
 
 #include <stdint.h>
 
+#include "mp_rn.h"   /* the 32 x 32 + 96-bit multiply-accumulates */
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define GF_HD __host__ __device__ __forceinline__
@@ -145,6 +147,44 @@ GF_HD limb pick(const limb (&a)[L], int i)
     return x;
 }
 
+/* Z (2L limbs, most significant first) = U * V, where U and V (L limbs,
+   most significant first) are zero below their top K limbs: the 2K-word
+   numbers of those limbs (32-bit words, least significant first) multiplied
+   column by column with a 96-bit accumulator, one v_mad_u64_u32 + v_addc per
+   word product; the 2(L - K) low limbs of Z are zero. */
+template <int L, int K>
+GF_HD void word_product(const limb (&U)[L], const limb (&V)[L], limb (&Z)[2 * L])
+{
+    constexpr int W = 2 * K;
+    uint32_t UW[W], VW[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int li = K - 1 - (k >> 1);
+        UW[k] = (k & 1) ? (uint32_t)(U[li] >> 32) : (uint32_t)U[li];
+        VW[k] = (k & 1) ? (uint32_t)(V[li] >> 32) : (uint32_t)V[li];
+    }
+    uint32_t ZW[2 * W];
+    uint64_t acc = 0;
+    uint32_t c2 = 0;
+#pragma unroll
+    for (int q = 0; q < 2 * W - 1; ++q) {
+        const int i0 = q < W ? 0 : q - W + 1;
+        const int i1 = q < W ? q : W - 1;
+        mdzg::mac96_first(UW[i0], VW[q - i0], acc, c2);
+#pragma unroll
+        for (int i = i0 + 1; i <= i1; ++i)
+            mdzg::mac96(UW[i], VW[q - i], acc, c2);
+        ZW[q] = (uint32_t)acc;
+        acc = (acc >> 32) | ((uint64_t)c2 << 32);
+    }
+    ZW[2 * W - 1] = (uint32_t)acc;
+#pragma unroll
+    for (int m = 0; m < 2 * L; ++m) {
+        const int k = 2 * (2 * K - 1 - m);
+        Z[m] = (m < 2 * K) ? ((limb)ZW[k] | ((limb)ZW[k + 1] << 32)) : (limb)0;
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* mpf_mul (mpf/mul.c): each operand cut to its top PREC limbs, the exact
    product of those, a zero top limb dropped, the top PREC + 1 limbs kept.   */
@@ -161,32 +201,15 @@ GF_HD void gf_mul(gf<L>& r, const gf<L>& u, const gf<L>& v, int lr)
         U[i] = (i < us) ? u.t[i] : (limb)0;
         V[i] = (i < vs) ? v.t[i] : (limb)0;
     }
-    /* product of U and V as L-limb numbers, column by column from the least
-       significant limb; Z is the 2L-limb product, most significant first */
+    /* U and V hold at most PREC <= L - 1 limbs, so their last limb is zero;
+       the product of their top K limbs is formed by word_product.  K = L - 1
+       always suffices; the small buckets, which often hold fewer limbs than
+       they store, take K = L - 2 when no lane of the wave needs more. */
     limb Z[2 * L];
-    limb a0 = 0, a1 = 0, a2 = 0;       /* 192-bit column accumulator */
-#pragma unroll
-    for (int q = 0; q < 2 * L - 1; ++q) {     /* q: LS-first column index */
-#pragma unroll
-        for (int i = 0; i < L; ++i) {
-            const int j = q - i;
-            /* U[L-1-i] and V[L-1-j] are zero unless L-1-i < us and L-1-j < vs;
-               in the small buckets, which often hold fewer limbs than they
-               store, a wave skips the products that are zero on all lanes */
-            if (j >= 0 && j < L && (L > 6 || gf_any((L - 1 - i < us) & (L - 1 - j < vs)))) {
-                limb hi, lo, c;
-                mul64(U[L - 1 - i], V[L - 1 - j], hi, lo);
-                a0 = addc(a0, lo, 0, c);
-                a1 = addc(a1, hi, c, c);
-                a2 += c;
-            }
-        }
-        Z[2 * L - 1 - q] = a0;
-        a0 = a1;
-        a1 = a2;
-        a2 = 0;
-    }
-    Z[0] = a0;
+    if (L <= 6 && L >= 3 && !gf_any((us > L - 2) | (vs > L - 2)))
+        word_product<L, (L >= 3 ? L - 2 : 1)>(U, V, Z);
+    else
+        word_product<L, L - 1>(U, V, Z);
     const bool zero = (us == 0) | (vs == 0);
     const int adj = (Z[0] == 0) ? 1 : 0;
     const int rsize = us + vs - adj;

@@ -53,8 +53,10 @@ int mdzg_class_of(int path, long precision, int* nw, int* kclr)
             }
         return -1;
     }
+    /* p >= 2: MPFR builds older than 4.0 have no 1-bit precision, and the
+       device arithmetic is checked from 2 bits up */
     long p = (path == MDZG_PATH_LD) ? 64 : precision;
-    if (p < 1 || p > MDZG_MAX_PRECISION)
+    if (p < 2 || p > MDZG_MAX_PRECISION)
         return -1;
     for (int i = 0; i < MDZG_NCLASS_N; ++i) {
         if (32L * mdzg_nwords[i] >= p) {

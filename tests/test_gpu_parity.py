@@ -306,3 +306,52 @@ def test_async_submit_and_cancel(ctx):
     rc2, st2 = job2.wait()
     job2.free()
     assert rc2 == 0 and st2["pixel_iters"] > 0
+
+
+def test_rejects_bad_views_and_tiles(ctx):
+    """Views and tiles outside what the ABI accepts are errors, not renders."""
+    base = dict(precision=128, depth=100, width=16, height=12)
+    rect = ("-2", "2", "1.5", "4")
+
+    def code(rect=rect, lines=None, cols=None, **kw):
+        args = dict(base)
+        args.update(kw)
+        with pytest.raises(M.MdzgError) as e:
+            ctx.render(M.make_view(rect, **args), lines=lines, cols=cols)
+        return e.value.code
+
+    assert code(precision=1025) == -7
+    assert code(precision=1) == -7
+    assert code(precision=1025, path=M.PATH_GMP) == -7
+    assert code(depth=0) == -1
+    assert code(depth=2 ** 31) == -1
+    assert code(fractal=4) == -1
+    assert code(family=2) == -1
+    assert code(path=4) == -1
+    assert code(lines=(10, 3)) == -1                       # rows beyond the frame
+    assert code(cols=(0, 17)) == -1
+    assert code(rect=("x", "2", "1.5", "4")) == -6
+    assert code(family=M.FAMILY_JULIA) == -6               # no Julia constant
+
+
+@pytest.mark.parametrize("path", [M.PATH_MPFR, M.PATH_LD, M.PATH_DOUBLE, M.PATH_GMP])
+def test_extreme_but_valid_views(ctx, oracle, path):
+    """1x1 frames, depth 1, depth INT32_MAX on escaping pixels, one long row."""
+    p = 2 if path in (M.PATH_MPFR, M.PATH_GMP) else 64
+    v = M.make_view(("2", "3", "0", "1"), precision=p, depth=2 ** 31 - 1, width=1, height=1,
+                    path=path)
+    got, st = ctx.render(v)
+    assert got.tolist() == [[1]] and st["pixel_iters"] == 1.0      # c = 2: |z2|^2 = 36 > 4
+    v = M.make_view(("-2", "2", "1.5", "4"), precision=p, depth=1, width=40, height=30,
+                    path=path)
+    got, _ = ctx.render(v)
+    ref = oracle.render(["-2", "2", "1.5", "4"], precision=p, depth=1, width=40, height=30,
+                        path=path, threads=8)
+    assert _eq(got, ref)
+    W = 20000
+    v = M.make_view(("-2", "1", "0.0001", "3"), precision=p, depth=60, width=W, height=3,
+                    path=path)
+    got, _ = ctx.render(v, lines=(1, 1))
+    ref = oracle.render(["-2", "1", "0.0001", "3"], precision=p, depth=60, width=W, height=3,
+                        path=path, lines=(1, 1, 1), threads=16)
+    assert _eq(got, ref)
