@@ -52,8 +52,8 @@ __device__ __forceinline__ void gmp_init(gmp_state<L>& s, const mdzg_dtile& t, i
     const int vs = t.vs, lr = t.gprec;
     load_gf<L>(s.wre, t.xs + (size_t)col * vs);
     load_gf<L>(s.wim, t.ys + (size_t)row * vs);
-    gf_mul<L>(s.wre2, s.wre, s.wre, lr);
-    gf_mul<L>(s.wim2, s.wim, s.wim, lr);
+    gf_sqr<L>(s.wre2, s.wre, lr);
+    gf_sqr<L>(s.wim2, s.wim, lr);
     if (t.family == 1) {
         load_gf<L>(s.cre, t.jc);
         load_gf<L>(s.cim, t.jc + vs);
@@ -77,8 +77,8 @@ __device__ __forceinline__ bool gmp_step(gmp_state<L>& s, int fractal, int32_t w
     if (fractal == F_GENERALIZED_CELTIC || (fractal == F_VARIANT && (wz & 1)))
         gf_abs<L>(t1);
     gf_add<L>(s.wre, t1, s.cre, 0u, lr);                   /* wre = t1 + c_re    */
-    gf_mul<L>(s.wim2, s.wim, s.wim, lr);                   /* wim2 = wim^2       */
-    gf_mul<L>(s.wre2, s.wre, s.wre, lr);                   /* wre2 = wre^2       */
+    gf_sqr<L>(s.wim2, s.wim, lr);                          /* wim2 = wim^2       */
+    gf_sqr<L>(s.wre2, s.wre, lr);                          /* wre2 = wre^2       */
     gf_add<L>(t1, s.wim2, s.wre2, 0u, lr);                 /* mpf_cmp (.., 4)    */
     return gf_gt4<L>(t1);
 }

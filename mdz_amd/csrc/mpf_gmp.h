@@ -185,11 +185,67 @@ GF_HD void word_product(const limb (&U)[L], const limb (&V)[L], limb (&Z)[2 * L]
     }
 }
 
+/* Z = U * U as word_product, with each off-diagonal word product formed
+   once and the column's off-diagonal sum doubled (pays from 12 words up) */
+template <int L, int K>
+GF_HD void word_square(const limb (&U)[L], limb (&Z)[2 * L])
+{
+    constexpr int W = 2 * K;
+    uint32_t UW[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int li = K - 1 - (k >> 1);
+        UW[k] = (k & 1) ? (uint32_t)(U[li] >> 32) : (uint32_t)U[li];
+    }
+    uint32_t ZW[2 * W];
+    uint64_t acc = 0;
+    uint32_t c2 = 0;
+#pragma unroll
+    for (int q = 0; q < 2 * W - 1; ++q) {
+        /* o = sum of UW[i] UW[q - i] over i < q - i (96 bits) */
+        uint64_t o = 0;
+        uint32_t o2 = 0;
+        bool first = true;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            const int j = q - i;
+            if (j > i && j < W) {
+                if (first)
+                    mdzg::mac96_first(UW[i], UW[j], o, o2);
+                else
+                    mdzg::mac96(UW[i], UW[j], o, o2);
+                first = false;
+            }
+        }
+        if (!first) {                  /* acc += 2 o */
+            const uint32_t d0 = (uint32_t)o << 1;
+            const uint32_t d1 = mdzg::funnel_r((uint32_t)(o >> 32), (uint32_t)o, 31u);
+            const uint32_t d2 = mdzg::funnel_r(o2, (uint32_t)(o >> 32), 31u);
+            uint32_t cc;
+            const uint32_t l0 = mdzg::add_cc((uint32_t)acc, d0, 0u, cc);
+            const uint32_t l1 = mdzg::add_cc((uint32_t)(acc >> 32), d1, cc, cc);
+            c2 = c2 + d2 + cc;
+            acc = ((uint64_t)l1 << 32) | l0;
+        }
+        if ((q & 1) == 0)
+            mdzg::mac96(UW[q >> 1], UW[q >> 1], acc, c2);
+        ZW[q] = (uint32_t)acc;
+        acc = (acc >> 32) | ((uint64_t)c2 << 32);
+        c2 = 0;
+    }
+    ZW[2 * W - 1] = (uint32_t)acc;
+#pragma unroll
+    for (int m = 0; m < 2 * L; ++m) {
+        const int k = 2 * (2 * K - 1 - m);
+        Z[m] = (m < 2 * K) ? ((limb)ZW[k] | ((limb)ZW[k + 1] << 32)) : (limb)0;
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* mpf_mul (mpf/mul.c): each operand cut to its top PREC limbs, the exact
    product of those, a zero top limb dropped, the top PREC + 1 limbs kept.   */
 /* ------------------------------------------------------------------ */
-template <int L>
+template <int L, bool SQR = false>
 GF_HD void gf_mul(gf<L>& r, const gf<L>& u, const gf<L>& v, int lr)
 {
     const int P = lr - 1;
@@ -206,7 +262,9 @@ GF_HD void gf_mul(gf<L>& r, const gf<L>& u, const gf<L>& v, int lr)
        always suffices; the small buckets, which often hold fewer limbs than
        they store, take K = L - 2 when no lane of the wave needs more. */
     limb Z[2 * L];
-    if (L <= 6 && L >= 3 && !gf_any((us > L - 2) | (vs > L - 2)))
+    if (SQR && 2 * (L - 1) >= 12)
+        word_square<L, L - 1>(U, Z);
+    else if (L <= 6 && L >= 3 && !gf_any((us > L - 2) | (vs > L - 2)))
         word_product<L, (L >= 3 ? L - 2 : 1)>(U, V, Z);
     else
         word_product<L, L - 1>(U, V, Z);
@@ -219,6 +277,13 @@ GF_HD void gf_mul(gf<L>& r, const gf<L>& u, const gf<L>& v, int lr)
     r.size = keep;
     r.exp = zero ? 0 : u.exp + v.exp - adj;
     r.neg = zero ? 0 : (u.neg ^ v.neg);
+}
+
+/* mpf_mul (x, u, u) */
+template <int L>
+GF_HD void gf_sqr(gf<L>& r, const gf<L>& u, int lr)
+{
+    gf_mul<L, true>(r, u, u, lr);
 }
 
 /* ------------------------------------------------------------------ */

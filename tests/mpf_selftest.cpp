@@ -162,6 +162,8 @@ static void check(long cases)
         if (!same(r, ref)) report("mul", P, a, b, r, ref);
         gf_mul<L>(r, a, a, LR); mpf_mul(R, A, A); from_gmp(ref, R);
         if (!same(r, ref)) report("sqr", P, a, a, r, ref);
+        gf_sqr<L>(r, a, LR);
+        if (!same(r, ref)) report("sqr-sym", P, a, a, r, ref);
         gf_mul2<L>(r, a, LR); mpf_mul_ui(R, A, 2); from_gmp(ref, R);
         if (!same(r, ref)) report("mul2", P, a, a, r, ref);
         {
@@ -229,8 +231,8 @@ static void check_chain(long cases)
             gf_sub<L>(t, zr2, zi2, L);
             if (fractal == 2 || (fractal == 3 && (wz % 2))) gf_abs<L>(t);
             gf_add<L>(zr, t, cr, 0u, L);
-            gf_mul<L>(zi2, zi, zi, L);
-            gf_mul<L>(zr2, zr, zr, L);
+            gf_sqr<L>(zi2, zi, L);
+            gf_sqr<L>(zr2, zr, L);
             gf_add<L>(s, zi2, zr2, 0u, L);
             const int esc = gf_gt4<L>(s) ? 1 : 0;
 
