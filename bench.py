@@ -133,7 +133,7 @@ def gmp_path(ctx, M, WL, a, cpu=True):
     r = dict(value=sum(pis) / el / 1e9, steps=steps, warmup=1, ms_per_step=1e3 * el / steps,
              dtype="u64 limbs, GMP mpf PREC 3 (p=128)",
              config=dict(workload="config2 view on the GMP path (truncating mpf, p=128) "
-                         "640x480 depth 5000", kernel=f"k_gmp_persist<6>"),
+                         "640x480 depth 5000", kernel="k_gmp_persist<4>"),
              kernel_gpix_per_s=pis[0] / (sum(kms) / len(kms) * 1e-3) / 1e9)
     log(f"gmp path: {r['kernel_gpix_per_s']:.2f} G pixel-iter/s (kernel)")
     if cpu:
