@@ -441,6 +441,20 @@ def main():
                            kernel_gpix_per_s=p1 / (k1a * 1e-3) / 1e9,
                            frac_fp64_peak=7 * p1 / (k1a * 1e-3) / 1e12 / PEAK_TOPS)
         log(f"fp64: {res['fp64']['kernel_gpix_per_s']:.1f} G pixel-iter/s (kernel)")
+        # the FP64 kernel on a frame that fills the device: the same view at 3840x2160
+        v4 = WL.config1_view(W=3840, H=2160)
+        ctx.render(v4)
+        k4 = []
+        for _ in range(5):
+            _, st = ctx.render(v4)
+            k4.append(st["kernel_ms"])
+        k4a = sum(k4) / len(k4)
+        p4 = st["pixel_iters"]
+        res["fp64"]["at_3840x2160"] = dict(pixel_iters=p4, kernel_ms=k4a,
+                                           kernel_gpix_per_s=p4 / (k4a * 1e-3) / 1e9,
+                                           frac_fp64_peak=7 * p4 / (k4a * 1e-3) / 1e12 / PEAK_TOPS)
+        log(f"fp64 3840x2160: {res['fp64']['at_3840x2160']['kernel_gpix_per_s']:.1f} G "
+            f"pixel-iter/s (kernel)")
 
         # 1024-bit figure: a band of config 5 rows
         res["rn1024"] = rn1024_band(ctx, WL, 192)

@@ -52,22 +52,43 @@ __device__ __forceinline__ int find_tile(const mdzg_dtile* tiles, int ntiles, ui
     return lo;
 }
 
-/* wave-aggregated claim of one queue slot per requesting lane */
-__device__ __forceinline__ uint64_t claim(bool need, unsigned long long* counter, bool& got_any)
+/* Queue slots are claimed per wave in runs: one atomic on the shared counter
+   takes MDZG_CLAIM_RUN consecutive slots for the wave, which hands them to
+   its lanes in order as lanes need work.  A frame of quickly escaping pixels
+   refills lanes constantly; one atomic per refill from every wave made the
+   shared counter the bottleneck.  A run of one wave's width keeps the end of
+   a small frame spread over the device. */
+#define MDZG_CLAIM_RUN 64u
+struct wave_run {
+    uint64_t base;      /* next slot of the run (the same on every lane) */
+    uint32_t left;      /* slots left in the run */
+};
+
+__device__ __forceinline__ uint64_t claim(bool need, unsigned long long* counter, wave_run& w)
 {
     const uint64_t m = __ballot(need);
-    got_any = m != 0;
     uint64_t g = ~0ull;
     if (m) {
         const int lane = __lane_id();
-        const int leader = __ffsll((long long)m) - 1;
-        unsigned long long base = 0;
-        if (lane == leader)
-            base = atomicAdd(counter, (unsigned long long)__popcll(m));
-        base = __shfl(base, leader);
-        const uint64_t below = m & ((1ull << lane) - 1ull);
-        if (need)
-            g = base + (uint64_t)__popcll(below);
+        const uint32_t n = (uint32_t)__popcll(m);
+        const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (w.left < n) {
+            /* the rest of the run, then a new run (n <= 64 <= its length) */
+            const int leader = __ffsll((long long)m) - 1;
+            unsigned long long nb = 0;
+            if (lane == leader)
+                nb = atomicAdd(counter, (unsigned long long)MDZG_CLAIM_RUN);
+            nb = __shfl(nb, leader);
+            g = below < w.left ? w.base + below : nb + (below - w.left);
+            w.base = nb + (n - w.left);
+            w.left = MDZG_CLAIM_RUN - (n - w.left);
+        } else {
+            g = w.base + below;
+            w.base += n;
+            w.left -= n;
+        }
+        if (!need)
+            g = ~0ull;
     }
     return g;
 }
@@ -149,11 +170,11 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
     fp64_state s;
     s.wre = s.wim = s.wre2 = s.wim2 = s.cre = s.cim = 0.0;
     bool drained = false;
+    wave_run run = { 0, 0u };
     for (;;) {
         if (!drained) {
             const bool need = !active;
-            bool any;
-            const uint64_t g = claim(need, counter, any);
+            const uint64_t g = claim(need, counter, run);
             if (need && g < total) {
                 const int ti = find_tile(tiles, ntiles, g);
                 const mdzg_dtile& t = tiles[ti];
@@ -206,11 +227,11 @@ __global__ __launch_bounds__(256) void k_fp64_slice(const mdzg_dtile* tiles, int
     fp64_state s;
     s.wre = s.wim = s.wre2 = s.wim2 = s.cre = s.cim = 0.0;
     bool drained = false;
+    wave_run run = { 0, 0u };
     for (;;) {
         if (!drained) {
           const bool need = !active;
-          bool any;
-          const uint64_t g = claim(need, counter, any);
+          const uint64_t g = claim(need, counter, run);
           if (need && g < total) {
             const int ti = find_tile(tiles, ntiles, g);
             tp = &tiles[ti];
@@ -432,11 +453,11 @@ __global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_persist(c
     rn_state<N> s;
     s.c.init(c_lds);
     bool drained = false;
+    wave_run run = { 0, 0u };
     for (;;) {
         if (!drained) {
             const bool need = !active;
-            bool any;
-            const uint64_t g = claim(need, counter, any);
+            const uint64_t g = claim(need, counter, run);
             if (need && g < total) {
                 const int ti = find_tile(tiles, ntiles, g);
                 const mdzg_dtile& t = tiles[ti];
@@ -494,11 +515,11 @@ __global__ __launch_bounds__(256, (rn_min_waves_slice<N, FULL>())) void k_rn_sli
     rn_state<N> s;
     s.c.init(c_lds);
     bool drained = false;
+    wave_run run = { 0, 0u };
     for (;;) {
         if (!drained) {
           const bool need = !active;
-          bool any;
-          const uint64_t g = claim(need, counter, any);
+          const uint64_t g = claim(need, counter, run);
           if (need && g < total) {
             const int ti = find_tile(tiles, ntiles, g);
             tp = &tiles[ti];

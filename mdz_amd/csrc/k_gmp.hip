@@ -118,11 +118,11 @@ __global__ __launch_bounds__(256) void k_gmp_persist(const mdzg_dtile* tiles, in
     uint32_t* out = nullptr;
     gmp_state<L> s;
     bool drained = false;
+    wave_run run = { 0, 0u };
     for (;;) {
         if (!drained) {
             const bool need = !active;
-            bool any;
-            const uint64_t g = claim(need, counter, any);
+            const uint64_t g = claim(need, counter, run);
             if (need && g < total) {
                 const int ti = find_tile(tiles, ntiles, g);
                 const mdzg_dtile& t = tiles[ti];
@@ -170,11 +170,11 @@ __global__ __launch_bounds__(256) void k_gmp_slice(const mdzg_dtile* tiles, int 
     const mdzg_dtile* tp = tiles;
     gmp_state<L> s;
     bool drained = false;
+    wave_run run = { 0, 0u };
     for (;;) {
         if (!drained) {
             const bool need = !active;
-            bool any;
-            const uint64_t g = claim(need, counter, any);
+            const uint64_t g = claim(need, counter, run);
             if (need && g < total) {
                 const int ti = find_tile(tiles, ntiles, g);
                 tp = &tiles[ti];

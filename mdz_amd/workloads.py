@@ -27,8 +27,8 @@ PREC_CLASSES = (53, 64, 128, 256, 512, 1024)     # 53 = the FP64 class
 CONFIG3_SEED = 0x4D445A
 
 
-def config1_view(fractal=MANDELBROT, path=PATH_DOUBLE):
-    W, H, width = 240, 180, Decimal(3)
+def config1_view(fractal=MANDELBROT, path=PATH_DOUBLE, W=240, H=180):
+    width = Decimal(3)
     s = width / W
     xmin = Decimal("-0.75") - Decimal(W - 1) / 2 * s
     ymax = Decimal(0) + Decimal(H - 1) / 2 * s
