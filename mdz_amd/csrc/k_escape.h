@@ -52,18 +52,21 @@ __device__ __forceinline__ int find_tile(const mdzg_dtile* tiles, int ntiles, ui
     return lo;
 }
 
-/* Queue slots are claimed per wave in runs: one atomic on the shared counter
-   takes MDZG_CLAIM_RUN consecutive slots for the wave, which hands them to
-   its lanes in order as lanes need work.  A frame of quickly escaping pixels
-   refills lanes constantly; one atomic per refill from every wave made the
-   shared counter the bottleneck.  A run of one wave's width keeps the end of
-   a small frame spread over the device. */
-#define MDZG_CLAIM_RUN 64u
+/* Queue slots are claimed per wave.  A wave whose lanes need n slots takes
+   what is left of its run first; the rest comes from one atomic on the
+   shared counter, which takes max(missing, RUN) consecutive slots, the
+   surplus kept as the wave's run for later refills.  For the cheap loops
+   (FP64, RN of 1-2 words) a frame of quickly escaping pixels refills lanes
+   constantly, and one atomic per refill from every wave made the shared
+   counter the bottleneck: RUN = 64.  For the wide RN and GMP loops a pixel
+   costs far more than an atomic, and a reserved run would lengthen the
+   frame's tail by a pixel's time: RUN = 0, one slot per lane as it frees. */
 struct wave_run {
     uint64_t base;      /* next slot of the run (the same on every lane) */
     uint32_t left;      /* slots left in the run */
 };
 
+template <uint32_t RUN>
 __device__ __forceinline__ uint64_t claim(bool need, unsigned long long* counter, wave_run& w)
 {
     const uint64_t m = __ballot(need);
@@ -73,15 +76,16 @@ __device__ __forceinline__ uint64_t claim(bool need, unsigned long long* counter
         const uint32_t n = (uint32_t)__popcll(m);
         const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         if (w.left < n) {
-            /* the rest of the run, then a new run (n <= 64 <= its length) */
+            const uint32_t miss = n - w.left;
+            const uint32_t take = miss > RUN ? miss : RUN;
             const int leader = __ffsll((long long)m) - 1;
             unsigned long long nb = 0;
             if (lane == leader)
-                nb = atomicAdd(counter, (unsigned long long)MDZG_CLAIM_RUN);
+                nb = atomicAdd(counter, (unsigned long long)take);
             nb = __shfl(nb, leader);
             g = below < w.left ? w.base + below : nb + (below - w.left);
-            w.base = nb + (n - w.left);
-            w.left = MDZG_CLAIM_RUN - (n - w.left);
+            w.base = nb + miss;
+            w.left = take - miss;
         } else {
             g = w.base + below;
             w.base += n;
@@ -91,6 +95,13 @@ __device__ __forceinline__ uint64_t claim(bool need, unsigned long long* counter
             g = ~0ull;
     }
     return g;
+}
+
+/* run length by loop: FP64 and RN up to 2 words claim in runs of 64 */
+template <int N>
+constexpr uint32_t claim_run()
+{
+    return N <= 2 ? 64u : 0u;
 }
 
 #ifdef MDZG_FP64_KERNELS
@@ -174,7 +185,7 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
     for (;;) {
         if (!drained) {
             const bool need = !active;
-            const uint64_t g = claim(need, counter, run);
+            const uint64_t g = claim<64u>(need, counter, run);
             if (need && g < total) {
                 const int ti = find_tile(tiles, ntiles, g);
                 const mdzg_dtile& t = tiles[ti];
@@ -231,7 +242,7 @@ __global__ __launch_bounds__(256) void k_fp64_slice(const mdzg_dtile* tiles, int
     for (;;) {
         if (!drained) {
           const bool need = !active;
-          const uint64_t g = claim(need, counter, run);
+          const uint64_t g = claim<64u>(need, counter, run);
           if (need && g < total) {
             const int ti = find_tile(tiles, ntiles, g);
             tp = &tiles[ti];
@@ -457,7 +468,7 @@ __global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_persist(c
     for (;;) {
         if (!drained) {
             const bool need = !active;
-            const uint64_t g = claim(need, counter, run);
+            const uint64_t g = claim<claim_run<N>()>(need, counter, run);
             if (need && g < total) {
                 const int ti = find_tile(tiles, ntiles, g);
                 const mdzg_dtile& t = tiles[ti];
@@ -519,7 +530,7 @@ __global__ __launch_bounds__(256, (rn_min_waves_slice<N, FULL>())) void k_rn_sli
     for (;;) {
         if (!drained) {
           const bool need = !active;
-          const uint64_t g = claim(need, counter, run);
+          const uint64_t g = claim<claim_run<N>()>(need, counter, run);
           if (need && g < total) {
             const int ti = find_tile(tiles, ntiles, g);
             tp = &tiles[ti];

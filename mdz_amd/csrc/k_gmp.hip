@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_gmp_persist(const mdzg_dtile* tiles, in
     for (;;) {
         if (!drained) {
             const bool need = !active;
-            const uint64_t g = claim(need, counter, run);
+            const uint64_t g = claim<0u>(need, counter, run);
             if (need && g < total) {
                 const int ti = find_tile(tiles, ntiles, g);
                 const mdzg_dtile& t = tiles[ti];
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_gmp_slice(const mdzg_dtile* tiles, int 
     for (;;) {
         if (!drained) {
             const bool need = !active;
-            const uint64_t g = claim(need, counter, run);
+            const uint64_t g = claim<0u>(need, counter, run);
             if (need && g < total) {
                 const int ti = find_tile(tiles, ntiles, g);
                 tp = &tiles[ti];
