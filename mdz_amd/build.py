@@ -35,7 +35,13 @@ C_FLAGS = [
     "-ffp-contract=off", "-pthread",
     "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{MP_PREFIX}/include",
 ]
-LINK_LIBS = [f"-L{MP_PREFIX}/lib", f"-Wl,-rpath,{MP_PREFIX}/lib", "-lmpfr", "-lgmp", "-lz",
+# MPFR, GMP and zlib are linked from the system library directory, by
+# soname, with no run path: a C program (MDZ itself) that links libmdzg.so
+# then resolves the C++ runtime the ROCm libraries need from the system, not
+# from a prefix that ships an older one.  The headers come from MP_PREFIX;
+# the sonames (libmpfr.so.6, libgmp.so.10) fix the ABI.
+SYS_LIB = os.environ.get("MDZG_SYS_LIBDIR", "/usr/lib/x86_64-linux-gnu")
+LINK_LIBS = [f"-L{SYS_LIB}", "-l:libmpfr.so.6", "-l:libgmp.so.10", "-l:libz.so.1",
              "-lpthread", "-lm"]
 
 

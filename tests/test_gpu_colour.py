@@ -123,3 +123,26 @@ def test_render_image_rejects_bad_arguments(ctx):
         ctx.render_image(v, 1, bad)
     with pytest.raises(M.MdzgError):
         ctx.colour(np.zeros((4, 4), dtype=np.uint32), 2, bad)
+
+
+def test_c_program_renders_a_scene_to_png(ctx, tmp_path):
+    """tests/capi/render_mdz.c, the C program of INTEGRATION's standalone
+    example, run as its own process: its PNG equals the image the Python
+    binding renders for the same file."""
+    import subprocess
+    from tests.test_colour_cpu import png_decode, _rgb_bytes
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "tests", "capi", "render_mdz")
+    if not os.path.exists(exe):
+        from tests.capi import build as B
+        B.build()
+    f = os.path.join(MDZ, "honeytrace.mdz")
+    out = tmp_path / "h.png"
+    print("render_mdz honeytrace 160 aa 2", flush=True)
+    r = subprocess.run([exe, f, "160", str(out), "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    v = M.scene_view(f, 160, aa=2)
+    rgb, _ = ctx.render_image(v, 2, M.scene_palette(f))
+    w, h, img = png_decode(str(out))
+    assert (w, h) == (rgb.shape[1], rgb.shape[0])
+    assert np.array_equal(img, _rgb_bytes(rgb))
