@@ -355,3 +355,24 @@ def test_extreme_but_valid_views(ctx, oracle, path):
     ref = oracle.render(["-2", "1", "0.0001", "3"], precision=p, depth=60, width=W, height=3,
                         path=path, lines=(1, 1, 1), threads=16)
     assert _eq(got, ref)
+
+
+def test_config5_band_at_full_width(ctx, oracle):
+    """Config 5 at its real row width (3840 samples, 1024 bits, depth 20000):
+    a 16-row band rendered whole, as four column tiles in one persistent
+    batch, and as four launches, gives one map; pixels spread over the band
+    equal the oracle's."""
+    v = WL.config5_view()
+    l0, nl = 1048, 16
+    whole, _ = ctx.render(v, lines=(l0, nl))
+    specs = [(v, (l0, nl), (c, 960)) for c in range(0, 3840, 960)]
+    parts, _ = ctx.render_batch(specs, M.BATCH_PERSISTENT)
+    assert _eq(whole, np.hstack(parts))
+    parts, _ = ctx.render_batch(specs, M.BATCH_PER_TILE)
+    assert _eq(whole, np.hstack(parts))
+    rng = random.Random(5)
+    for _ in range(6):
+        r, c = rng.randrange(nl), rng.randrange(3840)
+        ref = oracle.render(_rect(v), precision=1024, depth=20000, width=3840, height=2160,
+                            lines=(l0 + r, 1, 1), cols=(c, 1), threads=1)
+        assert int(whole[r, c]) == int(ref[0, 0]), (r, c)
