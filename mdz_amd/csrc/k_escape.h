@@ -168,6 +168,26 @@ __global__ __launch_bounds__(256) void k_fp64_tile(const mdzg_dtile* tp)
     t.out[idx] = count;
 }
 
+/* `chunk` iterations of formula F (a compile-time constant) for every lane of
+   the wave, without branches: a lane that finishes records its count in res
+   and stops advancing wz; it keeps computing on its last values until the
+   wave refills it.  The per-lane branches of the generic loop cost more
+   scalar exec-mask instructions per iteration than the loop has VALU
+   instructions, which bound FP64 at the scalar unit's rate. */
+template <int F>
+__device__ __forceinline__ void fp64_chunk(fp64_state& s, int32_t& wz, bool& active,
+                                           uint32_t& res, int32_t depth, int chunk)
+{
+    for (int it = 0; it < chunk; ++it) {
+        const int32_t w = wz + 1;
+        const bool esc = fp64_step(s, F, w);
+        const bool fin = active & (esc | (w >= depth));
+        res = fin ? (esc ? (uint32_t)w : 0u) : res;
+        wz = active ? w : wz;
+        active = active & !fin;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, int ntiles,
                                                       uint64_t total,
                                                       unsigned long long* counter,
@@ -208,13 +228,32 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
         /* cancel flag (device memory, written by the host on another stream):
            loaded before the chunk, tested after it, every 16th round */
         const int stop = ((++round & 15) == 0) ? *cancel : 0;
-        for (int it = 0; it < chunk; ++it) {
-            if (active) {
-                ++wz;
-                const bool esc = fp64_step(s, fractal, wz);
-                if (esc || wz >= depth) {
-                    *out = esc ? (uint32_t)wz : 0u;
-                    active = false;
+        /* one formula for the whole wave (the usual case): a branch-free chunk */
+        const uint64_t am = __ballot(active);
+        const int lead = __ffsll((long long)am) - 1;
+        const int f0 = __shfl(fractal, lead);
+        if (__all(!active || fractal == f0)) {
+            const bool was = active;
+            uint32_t res = 0;
+            switch (f0) {
+            case F_MANDELBROT: fp64_chunk<F_MANDELBROT>(s, wz, active, res, depth, chunk); break;
+            case F_BURNING_SHIP: fp64_chunk<F_BURNING_SHIP>(s, wz, active, res, depth, chunk); break;
+            case F_GENERALIZED_CELTIC:
+                fp64_chunk<F_GENERALIZED_CELTIC>(s, wz, active, res, depth, chunk);
+                break;
+            default: fp64_chunk<F_VARIANT>(s, wz, active, res, depth, chunk); break;
+            }
+            if (was && !active)
+                *out = res;
+        } else {
+            for (int it = 0; it < chunk; ++it) {
+                if (active) {
+                    ++wz;
+                    const bool esc = fp64_step(s, fractal, wz);
+                    if (esc || wz >= depth) {
+                        *out = esc ? (uint32_t)wz : 0u;
+                        active = false;
+                    }
                 }
             }
         }
