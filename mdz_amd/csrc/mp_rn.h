@@ -174,10 +174,34 @@ MP_HD void round_full(uint32_t* R, uint32_t G, uint32_t S, int32_t& e)
     add_ulp<N>(R, e, inc);
 }
 
+/* true when the predicate holds on every lane of the wave (device) / on
+   this value (host) */
+MP_HD bool wave_all(bool c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ballot_w64(!c) == 0;
+#else
+    return c;
+#endif
+}
+
 /* kclr = 32N - p in [1, 32N-1]: clear the low kclr bits of R, round at bit kclr */
 template <int N>
 MP_HD void round_masked(uint32_t* R, uint32_t G, uint32_t S, int32_t& e, int kclr)
 {
+    if (N == 1 || wave_all(kclr < 32)) {
+        /* the rounding position lies in the lowest word (p = 80, 100, 176,
+           184, ...; always for N = 1, where p >= 2): only word 0 is masked,
+           the increment ripples upward as in add_ulp */
+        const uint32_t rm = 1u << (kclr - 1);            /* round bit */
+        const uint32_t rbit = R[0] & rm;
+        const uint32_t st = G | S | (R[0] & (rm - 1u));  /* sticky    */
+        const uint32_t lsb = R[0] & (rm << 1);           /* kept lsb  */
+        R[0] &= ~((rm << 1) - 1u);
+        const uint32_t inc = (uint32_t)(rbit != 0) & (uint32_t)((st != 0) | (lsb != 0));
+        add_ulp<N>(R, e, inc << kclr);
+        return;
+    }
     const int wi = kclr >> 5, bi = kclr & 31;            /* lsb of the kept part */
     const int rr = kclr - 1, rw = rr >> 5, rb = rr & 31; /* round bit            */
     uint32_t rbit = 0, st = G | S, lsb = 0;
