@@ -52,6 +52,24 @@ __device__ __forceinline__ int find_tile(const mdzg_dtile* tiles, int ntiles, ui
     return lo;
 }
 
+/* Counts are stored and the cancel flag is read through global (address
+   space 1) pointers.  Through a generic pointer they compile to flat
+   instructions, which count against the LDS counter as well as the memory
+   counter, so each lane shuffle of the next refill round would wait until the
+   previous round's count stores had reached memory. */
+typedef __attribute__((address_space(1))) uint32_t g_u32;
+typedef __attribute__((address_space(1))) const volatile int g_cflag;
+
+__device__ __forceinline__ void put_count(uint32_t* p, uint32_t v)
+{
+    *(g_u32*)p = v;
+}
+
+__device__ __forceinline__ int read_cancel(const volatile int* p)
+{
+    return *(g_cflag*)p;
+}
+
 /* Queue slots are claimed per wave.  A wave whose lanes need n slots takes
    what is left of its run first; the rest comes from one atomic on the
    shared counter, which takes max(missing, RUN) consecutive slots, the
@@ -66,8 +84,8 @@ struct wave_run {
     uint32_t left;      /* slots left in the run */
 };
 
-template <uint32_t RUN>
-__device__ __forceinline__ uint64_t claim(bool need, unsigned long long* counter, wave_run& w)
+__device__ __forceinline__ uint64_t claim_len(bool need, unsigned long long* counter, wave_run& w,
+                                              uint32_t RUN)
 {
     const uint64_t m = __ballot(need);
     uint64_t g = ~0ull;
@@ -95,6 +113,31 @@ __device__ __forceinline__ uint64_t claim(bool need, unsigned long long* counter
             g = ~0ull;
     }
     return g;
+}
+
+template <uint32_t RUN>
+__device__ __forceinline__ uint64_t claim(bool need, unsigned long long* counter, wave_run& w)
+{
+    return claim_len(need, counter, w, RUN);
+}
+
+/* run length for a launch of `total` slots: a K-th of each wave's even share
+   of the queue, at least 64 and at most CAP.  Long runs save atomics on the
+   shared counter (the bottleneck of frames whose pixels escape within a few
+   iterations); capping a run at a fraction of the share bounds what one wave
+   can still hold when the queue drains, and small launches keep runs of 64 so
+   that every wave gets work. */
+#ifndef MDZG_RUN_K
+#define MDZG_RUN_K 4u
+#endif
+#ifndef MDZG_RUN_CAP
+#define MDZG_RUN_CAP 256u
+#endif
+__device__ __forceinline__ uint32_t share_run(uint64_t total)
+{
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    const uint64_t r = total / (waves * MDZG_RUN_K);
+    return r < 64u ? 64u : (r > MDZG_RUN_CAP ? MDZG_RUN_CAP : (uint32_t)r);
 }
 
 /* run length by loop: FP64 and RN up to 2 words claim in runs of 64 */
@@ -168,24 +211,53 @@ __global__ __launch_bounds__(256) void k_fp64_tile(const mdzg_dtile* tp)
     t.out[idx] = count;
 }
 
-/* `chunk` iterations of formula F (a compile-time constant) for every lane of
-   the wave, without branches: a lane that finishes records its count in res
-   and stops advancing wz; it keeps computing on its last values until the
-   wave refills it.  The per-lane branches of the generic loop cost more
-   scalar exec-mask instructions per iteration than the loop has VALU
-   instructions, which bound FP64 at the scalar unit's rate. */
-template <int F>
-__device__ __forceinline__ void fp64_chunk(fp64_state& s, int32_t& wz, bool& active,
-                                           uint32_t& res, int32_t depth, int chunk)
+/* v + 1 on the lanes set in the lane mask m, v elsewhere: one
+   v_addc_co_u32 with m as the carry-in */
+__device__ __forceinline__ int32_t add_lanes(int32_t v, uint64_t m)
 {
-    for (int it = 0; it < chunk; ++it) {
-        const int32_t w = wz + 1;
-        const bool esc = fp64_step(s, F, w);
-        const bool fin = active & (esc | (w >= depth));
-        res = fin ? (esc ? (uint32_t)w : 0u) : res;
-        wz = active ? w : wz;
-        active = active & !fin;
+    int32_t r;
+    uint64_t co;
+    asm("v_addc_co_u32 %0, %1, 0, %2, %3" : "=v"(r), "=s"(co) : "v"(v), "s"(m));
+    return r;
+}
+
+/* One iteration of formula F (a compile-time constant) for every lane of the
+   wave, without branches.  live is the lane mask of pixels still iterating:
+   their wz counts the iteration, and a lane that escapes leaves the mask, so
+   its wz stops at its count.  Other lanes compute on their last values, which
+   no one reads.  Per iteration that is the formula's FP64 operations, one
+   compare into a scalar mask and one add; the per-lane branches of the
+   generic loop cost more scalar exec-mask instructions per iteration than the
+   loop has VALU instructions. */
+template <int F>
+__device__ __forceinline__ void fp64_iter(fp64_state& s, int32_t& wz, uint64_t& live)
+{
+    const bool esc = fp64_step(s, F, wz + 1);
+    const uint64_t e = __builtin_amdgcn_ballot_w64(esc);
+    wz = add_lanes(wz, live);
+    live &= ~e;
+}
+
+/* n iterations of formula F, or fewer once MDZG_FP64_IDLE_EXIT of the lanes
+   live at the start (or all of them) have escaped, so that the wave refills
+   them.  The caller bounds n by every live lane's remaining depth, so no lane
+   passes its depth inside the run. */
+#ifndef MDZG_FP64_IDLE_EXIT
+#define MDZG_FP64_IDLE_EXIT 64
+#endif
+template <int F>
+__device__ __forceinline__ void fp64_run(fp64_state& s, int32_t& wz, uint64_t& live, int n)
+{
+    const int quit = __popcll(live) - MDZG_FP64_IDLE_EXIT;
+    int it = 0;
+    for (; it + 4 <= n && live != 0 && __popcll(live) > quit; it += 4) {
+        fp64_iter<F>(s, wz, live);
+        fp64_iter<F>(s, wz, live);
+        fp64_iter<F>(s, wz, live);
+        fp64_iter<F>(s, wz, live);
     }
+    for (; it < n; ++it)
+        fp64_iter<F>(s, wz, live);
 }
 
 __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, int ntiles,
@@ -202,10 +274,11 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
     s.wre = s.wim = s.wre2 = s.wim2 = s.cre = s.cim = 0.0;
     bool drained = false;
     wave_run run = { 0, 0u };
+    const uint32_t run_len = share_run(total);
     for (;;) {
         if (!drained) {
             const bool need = !active;
-            const uint64_t g = claim<64u>(need, counter, run);
+            const uint64_t g = claim_len(need, counter, run, run_len);
             if (need && g < total) {
                 const int ti = find_tile(tiles, ntiles, g);
                 const mdzg_dtile& t = tiles[ti];
@@ -218,6 +291,10 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
                 out = t.out + local;
                 wz = 0;
                 active = true;
+                /* the tile's fields are waited for here, on the refill path:
+                   a wait where the paths join would also wait, every round,
+                   for the count stores still in flight */
+                asm volatile("" ::"v"(fractal), "v"(depth), "v"(out));
             }
             /* the counter only grows: once one lane reads past the end the wave
                stops claiming, so no atomics while its last lanes finish */
@@ -227,31 +304,35 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
             break;
         /* cancel flag (device memory, written by the host on another stream):
            loaded before the chunk, tested after it, every 16th round */
-        const int stop = ((++round & 15) == 0) ? *cancel : 0;
-        /* one formula for the whole wave (the usual case): a branch-free chunk */
+        const int stop = ((++round & 15) == 0) ? read_cancel(cancel) : 0;
+        /* one formula for the whole wave (the usual case): a branch-free run
+           of at most `chunk` iterations, cut to the smallest remaining depth
+           of the wave's pixels */
         const uint64_t am = __ballot(active);
         const int lead = __ffsll((long long)am) - 1;
         const int f0 = __shfl(fractal, lead);
         if (__all(!active || fractal == f0)) {
-            const bool was = active;
-            uint32_t res = 0;
+            const int rem = __ockl_wfred_min_i32(active ? depth - wz : 0x7fffffff);
+            const int n = __builtin_amdgcn_readfirstlane(min(rem, chunk));
+            uint64_t live = am;
             switch (f0) {
-            case F_MANDELBROT: fp64_chunk<F_MANDELBROT>(s, wz, active, res, depth, chunk); break;
-            case F_BURNING_SHIP: fp64_chunk<F_BURNING_SHIP>(s, wz, active, res, depth, chunk); break;
-            case F_GENERALIZED_CELTIC:
-                fp64_chunk<F_GENERALIZED_CELTIC>(s, wz, active, res, depth, chunk);
-                break;
-            default: fp64_chunk<F_VARIANT>(s, wz, active, res, depth, chunk); break;
+            case F_MANDELBROT: fp64_run<F_MANDELBROT>(s, wz, live, n); break;
+            case F_BURNING_SHIP: fp64_run<F_BURNING_SHIP>(s, wz, live, n); break;
+            case F_GENERALIZED_CELTIC: fp64_run<F_GENERALIZED_CELTIC>(s, wz, live, n); break;
+            default: fp64_run<F_VARIANT>(s, wz, live, n); break;
             }
-            if (was && !active)
-                *out = res;
+            const bool esc = ((am & ~live) >> __lane_id()) & 1u;
+            if (active && (esc || wz >= depth)) {
+                put_count(out, esc ? (uint32_t)wz : 0u);
+                active = false;
+            }
         } else {
             for (int it = 0; it < chunk; ++it) {
                 if (active) {
                     ++wz;
                     const bool esc = fp64_step(s, fractal, wz);
                     if (esc || wz >= depth) {
-                        *out = esc ? (uint32_t)wz : 0u;
+                        put_count(out, esc ? (uint32_t)wz : 0u);
                         active = false;
                     }
                 }
@@ -316,7 +397,7 @@ __global__ __launch_bounds__(256) void k_fp64_slice(const mdzg_dtile* tiles, int
                 ++wz;
                 const bool esc = fp64_step(s, fractal, wz);
                 if (esc || wz >= depth) {
-                    tp->out[local] = esc ? (uint32_t)wz : 0u;
+                    put_count(tp->out + local, esc ? (uint32_t)wz : 0u);
                     atomicSub(tp->remaining, 1u);
                     active = false;
                 } else if (--left == 0) {
@@ -521,18 +602,19 @@ __global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_persist(c
                 out = t.out + local;
                 wz = 0;
                 active = true;
+                asm volatile("" ::"v"(fractal), "v"(kclr), "v"(depth), "v"(out));
             }
             drained = __any(need && g >= total);
         }
         if (!__any(active))
             break;
-        const int stop = ((++round & 15) == 0) ? *cancel : 0;
+        const int stop = ((++round & 15) == 0) ? read_cancel(cancel) : 0;
         for (int it = 0; it < chunk; ++it) {
             if (active) {
                 ++wz;
                 const bool esc = rn_step<N, FULL>(s, fractal, wz, kclr);
                 if (esc || wz >= depth) {
-                    *out = esc ? (uint32_t)wz : 0u;
+                    put_count(out, esc ? (uint32_t)wz : 0u);
                     active = false;
                 }
             }
@@ -614,7 +696,7 @@ __global__ __launch_bounds__(256, (rn_min_waves_slice<N, FULL>())) void k_rn_sli
             ++wz;
             const bool esc = rn_step<N, FULL>(s, fractal, wz, kclr);
             if (esc || wz >= depth) {
-                tp->out[local] = esc ? (uint32_t)wz : 0u;
+                put_count(tp->out + local, esc ? (uint32_t)wz : 0u);
                 atomicSub(tp->remaining, 1u);
                 active = false;
             } else if (--left == 0) {

@@ -136,18 +136,21 @@ __global__ __launch_bounds__(256) void k_gmp_persist(const mdzg_dtile* tiles, in
                 out = t.out + local;
                 wz = 0;
                 active = true;
+                /* wait for the tile's fields here, not where the paths join
+                   (there the wait would include the count stores in flight) */
+                asm volatile("" ::"v"(fractal), "v"(lr), "v"(depth), "v"(out));
             }
             drained = __any(need && g >= total);
         }
         if (!__any(active))
             break;
-        const int stop = ((++round & 15) == 0) ? *cancel : 0;
+        const int stop = ((++round & 15) == 0) ? read_cancel(cancel) : 0;
         for (int it = 0; it < chunk; ++it) {
             if (active) {
                 ++wz;
                 const bool esc = gmp_step<L>(s, fractal, wz, lr);
                 if (esc || wz >= depth) {
-                    *out = esc ? (uint32_t)wz : 0u;
+                    put_count(out, esc ? (uint32_t)wz : 0u);
                     active = false;
                 }
             }
@@ -218,7 +221,7 @@ __global__ __launch_bounds__(256) void k_gmp_slice(const mdzg_dtile* tiles, int 
             ++wz;
             const bool esc = gmp_step<L>(s, fractal, wz, lr);
             if (esc || wz >= depth) {
-                tp->out[local] = esc ? (uint32_t)wz : 0u;
+                put_count(tp->out + local, esc ? (uint32_t)wz : 0u);
                 atomicSub(tp->remaining, 1u);
                 active = false;
             } else if (--left == 0) {
