@@ -48,22 +48,42 @@ struct gf {
     uint32_t neg;
 };
 
+/* a + b + cin (cin 0 or 1), carry to cout.  On the device a limb chain is
+   one 32-bit v_add_co / v_addc_co chain (two instructions a limb) rather
+   than 64-bit adds with compares for the carries. */
 GF_HD limb addc(limb a, limb b, limb cin, limb& cout)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    unsigned int c0, c1;
+    const uint32_t lo = __builtin_addc((uint32_t)a, (uint32_t)b, (uint32_t)cin, &c0);
+    const uint32_t hi = __builtin_addc((uint32_t)(a >> 32), (uint32_t)(b >> 32), c0, &c1);
+    cout = c1;
+    return ((limb)hi << 32) | lo;
+#else
     const limb s = a + b;
     const limb c1 = (limb)(s < a);
     const limb r = s + cin;
     cout = c1 | (limb)(r < s);
     return r;
+#endif
 }
 
+/* a - b - bin (bin 0 or 1), borrow to bout (v_sub_co / v_subb_co) */
 GF_HD limb subb(limb a, limb b, limb bin, limb& bout)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    unsigned int b0, b1;
+    const uint32_t lo = __builtin_subc((uint32_t)a, (uint32_t)b, (uint32_t)bin, &b0);
+    const uint32_t hi = __builtin_subc((uint32_t)(a >> 32), (uint32_t)(b >> 32), b0, &b1);
+    bout = b1;
+    return ((limb)hi << 32) | lo;
+#else
     const limb d = a - b;
     const limb b1 = (limb)(a < b);
     const limb r = d - bin;
     bout = b1 | (limb)(d < bin);
     return r;
+#endif
 }
 
 GF_HD void mul64(limb a, limb b, limb& hi, limb& lo)
@@ -112,15 +132,18 @@ GF_HD void keep_top(limb (&a)[L], int n)
 }
 
 /* shift toward the less significant end by k limbs (k >= 0), zeros enter at
-   the top and limbs pushed past position L-1 are lost */
+   the top and limbs pushed past position L-1 are lost.  Exponent gaps are
+   mostly 0 or 1 limb, so a stage no lane of the wave needs is skipped. */
 template <int L>
 GF_HD void shift_down(limb (&a)[L], int k)
 {
 #pragma unroll
     for (int st = 1; st < 2 * L; st <<= 1) {
         const bool c = (k & st) != 0;
+        if (gf_any(c)) {
 #pragma unroll
-        for (int i = L - 1; i >= 0; --i) a[i] = c ? ((i >= st) ? a[i - st] : (limb)0) : a[i];
+            for (int i = L - 1; i >= 0; --i) a[i] = c ? ((i >= st) ? a[i - st] : (limb)0) : a[i];
+        }
     }
     if (k >= 2 * L) {
 #pragma unroll

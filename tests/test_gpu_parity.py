@@ -232,6 +232,47 @@ def test_batch_modes_agree_with_oracle(ctx, oracle):
         assert _eq(x, y) and _eq(x, r)
 
 
+@pytest.mark.parametrize("chunk", [1, 3, 4, 5, 256])
+def test_fp64_runs_mixed_depths(ctx, oracle, chunk):
+    """FP64 persistent runs: one launch whose waves hold lanes of small tiles
+    with depths 1 to 4097, all four formulas and a Julia tile, at run lengths
+    that use the unrolled part of a run, its single steps, or both.  Every map
+    equals the oracle and the one-launch-per-tile form."""
+    rect = _rect(WL.config1_view())
+    J = cases.JULIA
+    rng = random.Random(11)
+    specs, refs = [], []
+    for fractal, depth in ((0, 1), (0, 2), (1, 3), (2, 5), (3, 7), (0, 100), (3, 257),
+                           (1, 1000), (0, 4097)):
+        nl, nc = 7, 13
+        l0, c0 = rng.randrange(180 - nl), rng.randrange(240 - nc)
+        v = M.make_view(rect, precision=64, depth=depth, width=240, height=180,
+                        fractal=fractal, path=M.PATH_DOUBLE)
+        specs.append((v, (l0, nl), (c0, nc)))
+        refs.append(oracle.render(rect, precision=64, depth=depth, width=240, height=180,
+                                  fractal=fractal, path=M.PATH_DOUBLE, lines=(l0, nl, 1),
+                                  cols=(c0, nc), threads=8))
+    vj = M.make_view(J["rect"], precision=64, depth=J["depth"], width=96, height=72, fractal=3,
+                     family=M.FAMILY_JULIA, path=M.PATH_DOUBLE, julia=J["c"])
+    specs.append((vj, (30, 9), (40, 11)))
+    refs.append(oracle.render(J["rect"], precision=64, depth=J["depth"], width=96, height=72,
+                              fractal=3, family=M.FAMILY_JULIA, path=M.PATH_DOUBLE,
+                              julia=J["c"], lines=(30, 9, 1), cols=(40, 11), threads=8))
+    kid = M.kernel_id(M.PATH_DOUBLE, 64)
+    keep = ctx.tune_get(kid)[1]
+    ctx.tune_set(kid, chunk=chunk)
+    try:
+        a, sa = ctx.render_batch(specs, M.BATCH_PERSISTENT)
+        b, _ = ctx.render_batch(specs, M.BATCH_PER_TILE)
+    finally:
+        ctx.tune_set(kid, chunk=keep)
+    assert sa["launches"] == 1
+    for x, y, r in zip(a, b, refs):
+        assert _eq(x, r) and _eq(y, r)
+    assert sa["pixel_iters"] == sum(float(np.where(r > 0, r, s[0].depth).sum())
+                                    for r, s in zip(refs, specs))
+
+
 def test_tile_split_invariance(ctx):
     """A rectangle rendered whole equals the same rectangle rendered as four tiles."""
     rect, p, depth, fractal, path, (W, H) = cases.scene_rect("deep_embedded_julia")
