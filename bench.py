@@ -41,9 +41,12 @@ COLOUR_TRAFFIC_BYTES = {1: 66.4e6, 2: 166.0e6}
 CONFIG2_TRAFFIC_BYTES = 3.07e6
 # VALU instructions (wave64) per config 2 launch of k_rn_persist<4,true>:
 # rocprofv3 SQ_INSTS_VALU of the current kernel build
-CONFIG2_VALU_INSTS = 8.8672e9
+CONFIG2_VALU_INSTS = 8.8657e9
 # VALU issue limit: 256 CUs x one wave64 instruction per clock x 2.4 GHz peak clock
 PEAK_VALU_ISSUE = 256 * 2.4e9
+# VALU instructions per wave-iteration of the FP64 kernel's run loop: the
+# formula's 7 FP64 operations and the exact doubling, one compare, one add
+FP64_VALU_PER_ITER = 10
 
 
 def log(*a):
@@ -328,6 +331,26 @@ def config5_deep(ctx, M, WL, np, cpu=True):
     return out
 
 
+def fp64_interior(ctx, WL, reps=5):
+    """The FP64 kernel on a frame whose pixels all run to depth (1920x1080,
+    depth 2000, 4.15e9 pixel-iterations): the iteration rate without refills.
+    Its loop issues FP64_VALU_PER_ITER VALU instructions per wave-iteration,
+    which bounds the rate at PEAK_VALU_ISSUE * 64 / FP64_VALU_PER_ITER."""
+    v = WL.interior_view()
+    ctx.render(v)
+    ks = []
+    for _ in range(reps):
+        _, st = ctx.render(v)
+        ks.append(st["kernel_ms"])
+    k = sum(ks) / len(ks)
+    rate = st["pixel_iters"] / (k * 1e-3)
+    bound = PEAK_VALU_ISSUE * 64 / FP64_VALU_PER_ITER
+    return dict(workload="1920x1080 inside the main cardioid (-0.1 + 0.1i, width 1e-6), depth 2000",
+                pixel_iters=st["pixel_iters"], kernel_ms=k, kernel_gpix_per_s=rate / 1e9,
+                frac_fp64_peak=7 * rate / 1e12 / PEAK_TOPS,
+                valu_bound_gpix_per_s=bound / 1e9, frac_valu_bound=rate / bound)
+
+
 def rn1024_band(ctx, WL, rows):
     """Rows 1048.. of the config 5 frame (3840 samples each, p=1024, depth 20000)."""
     v5 = WL.config5_view()
@@ -351,6 +374,8 @@ def main():
                     help="only the config 2 timed steps (for counter runs)")
     ap.add_argument("--band-only", type=int, default=0, metavar="ROWS",
                     help="only the 1024-bit figure over ROWS rows of config 5 (for counter runs)")
+    ap.add_argument("--fp64-interior", action="store_true",
+                    help="only the FP64 all-interior frame (for counter runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only a plain device-to-device copy of MIB MiB (calibrates "
                          "FETCH_SIZE / WRITE_SIZE against known bytes)")
@@ -379,6 +404,11 @@ def main():
     ctx = M.Context(0)
     if a.band_only:
         r = rn1024_band(ctx, WL, a.band_only)
+        ctx.close()
+        print(json.dumps(r))
+        return
+    if a.fp64_interior:
+        r = fp64_interior(ctx, WL)
         ctx.close()
         print(json.dumps(r))
         return
@@ -454,6 +484,9 @@ def main():
                                            kernel_gpix_per_s=p4 / (k4a * 1e-3) / 1e9,
                                            frac_fp64_peak=7 * p4 / (k4a * 1e-3) / 1e12 / PEAK_TOPS)
         log(f"fp64 3840x2160: {res['fp64']['at_3840x2160']['kernel_gpix_per_s']:.1f} G "
+            f"pixel-iter/s (kernel)")
+        res["fp64"]["interior_1920x1080"] = fp64_interior(ctx, WL)
+        log(f"fp64 interior: {res['fp64']['interior_1920x1080']['kernel_gpix_per_s']:.1f} G "
             f"pixel-iter/s (kernel)")
 
         # 1024-bit figure: a band of config 5 rows

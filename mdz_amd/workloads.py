@@ -37,6 +37,14 @@ def config1_view(fractal=MANDELBROT, path=PATH_DOUBLE, W=240, H=180):
                      path=path)
 
 
+def interior_view(path=PATH_DOUBLE):
+    """A 1920x1080 frame inside the main cardioid (centre -0.1 + 0.1i, width
+    1e-6, depth 2000): every pixel runs to depth, so the frame measures the
+    iteration rate alone."""
+    rect = rect_from_center("-0.1", "0.1", "1e-6", 64, 1920, 1080)
+    return make_view(rect, precision=64, depth=2000, width=1920, height=1080, path=path)
+
+
 def config2_view():
     rect = rect_from_center(CONFIG2_CENTRE[0], CONFIG2_CENTRE[1], "1e-25", 128, 640, 480)
     return make_view(rect, precision=128, depth=5000, width=640, height=480)
