@@ -229,9 +229,7 @@ int mdzg_render_batch(mdzg_ctx* ctx, const mdzg_tile* tiles, int ntiles, int mod
    (0 = FP64; the RN kernels by word count and exact/ragged precision).
    blocks_per_cu: resident 256-thread blocks per CU the persistent grid is
    sized for (<= the kernel's occupancy limit, reported as max_blocks_per_cu);
-   chunk: iterations a lane runs between queue refills (the FP64 kernel runs
-   twice as many in a launch whose pixels all fit its lanes at once).  0 keeps
-   a value. */
+   chunk: iterations a lane runs between queue refills.  0 keeps a value. */
 int mdzg_kernel_id(int path, long precision);
 int mdzg_tune_set(mdzg_ctx* ctx, int kernel_id, int blocks_per_cu, int chunk);
 int mdzg_tune_get(mdzg_ctx* ctx, int kernel_id, int* blocks_per_cu, int* chunk,

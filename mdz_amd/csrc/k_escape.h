@@ -228,32 +228,56 @@ __device__ __forceinline__ int32_t add_lanes(int32_t v, uint64_t m)
    no one reads.  Per iteration that is the formula's FP64 operations, one
    compare into a scalar mask and one add; the per-lane branches of the
    generic loop cost more scalar exec-mask instructions per iteration than the
-   loop has VALU instructions. */
-template <int F>
-__device__ __forceinline__ void fp64_iter(fp64_state& s, int32_t& wz, uint64_t& live)
+   loop has VALU instructions.  With DEPTH a lane also leaves the mask at its
+   depth (one more compare), and escm collects the lanes that escaped. */
+template <int F, bool DEPTH>
+__device__ __forceinline__ void fp64_iter(fp64_state& s, int32_t& wz, uint64_t& live,
+                                          uint64_t& escm, int32_t depth)
 {
     const bool esc = fp64_step(s, F, wz + 1);
     const uint64_t e = __builtin_amdgcn_ballot_w64(esc);
     wz = add_lanes(wz, live);
-    live &= ~e;
+    if (DEPTH) {
+        escm |= e & live;
+        live &= ~(e | __builtin_amdgcn_ballot_w64(wz >= depth));
+    } else {
+        live &= ~e;
+    }
 }
 
-/* n iterations of formula F, or fewer once every lane has escaped.  The
-   caller bounds n by every live lane's remaining depth, so no lane passes its
-   depth inside the run.  (Leaving the run once a few lanes had escaped, to
-   refill them sooner, measured no faster.) */
-template <int F>
-__device__ __forceinline__ void fp64_run(fp64_state& s, int32_t& wz, uint64_t& live, int n)
+/* Up to n iterations of formula F, fewer once no lane is left iterating;
+   escm gets the lanes that escaped.  Without DEPTH the caller has checked
+   that no live lane reaches its depth within n iterations. */
+template <int F, bool DEPTH>
+__device__ __forceinline__ void fp64_run(fp64_state& s, int32_t& wz, uint64_t& live,
+                                         uint64_t& escm, int32_t depth, int n)
 {
+    const uint64_t start = live;
     int it = 0;
     for (; it + 4 <= n && live != 0; it += 4) {
-        fp64_iter<F>(s, wz, live);
-        fp64_iter<F>(s, wz, live);
-        fp64_iter<F>(s, wz, live);
-        fp64_iter<F>(s, wz, live);
+        fp64_iter<F, DEPTH>(s, wz, live, escm, depth);
+        fp64_iter<F, DEPTH>(s, wz, live, escm, depth);
+        fp64_iter<F, DEPTH>(s, wz, live, escm, depth);
+        fp64_iter<F, DEPTH>(s, wz, live, escm, depth);
     }
-    for (; it < n; ++it)
-        fp64_iter<F>(s, wz, live);
+    for (; it < n && live != 0; ++it)
+        fp64_iter<F, DEPTH>(s, wz, live, escm, depth);
+    if (!DEPTH)
+        escm = start & ~live;
+}
+
+template <bool DEPTH>
+__device__ __forceinline__ void fp64_run_formula(int f, fp64_state& s, int32_t& wz, uint64_t& live,
+                                                 uint64_t& escm, int32_t depth, int n)
+{
+    switch (f) {
+    case F_MANDELBROT: fp64_run<F_MANDELBROT, DEPTH>(s, wz, live, escm, depth, n); break;
+    case F_BURNING_SHIP: fp64_run<F_BURNING_SHIP, DEPTH>(s, wz, live, escm, depth, n); break;
+    case F_GENERALIZED_CELTIC:
+        fp64_run<F_GENERALIZED_CELTIC, DEPTH>(s, wz, live, escm, depth, n);
+        break;
+    default: fp64_run<F_VARIANT, DEPTH>(s, wz, live, escm, depth, n); break;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, int ntiles,
@@ -302,22 +326,23 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
            loaded before the chunk, tested after it, every 16th round */
         const int stop = ((++round & 15) == 0) ? read_cancel(cancel) : 0;
         /* one formula for the whole wave (the usual case): a branch-free run
-           of at most `chunk` iterations, cut to the smallest remaining depth
-           of the wave's pixels */
+           of `chunk` iterations.  When no pixel of the wave can reach its
+           depth within it (the wave minimum of the remaining depths), the
+           run skips the per-lane depth test.  Cutting the run at that
+           minimum instead made the runs of waves whose pixels started at
+           different times short: 1.60 against 1.82 T pixel-iter/s on the
+           3840x2160 config 1 frame. */
         const uint64_t am = __ballot(active);
         const int lead = __ffsll((long long)am) - 1;
         const int f0 = __shfl(fractal, lead);
         if (__all(!active || fractal == f0)) {
             const int rem = __ockl_wfred_min_i32(active ? depth - wz : 0x7fffffff);
-            const int n = __builtin_amdgcn_readfirstlane(min(rem, chunk));
-            uint64_t live = am;
-            switch (f0) {
-            case F_MANDELBROT: fp64_run<F_MANDELBROT>(s, wz, live, n); break;
-            case F_BURNING_SHIP: fp64_run<F_BURNING_SHIP>(s, wz, live, n); break;
-            case F_GENERALIZED_CELTIC: fp64_run<F_GENERALIZED_CELTIC>(s, wz, live, n); break;
-            default: fp64_run<F_VARIANT>(s, wz, live, n); break;
-            }
-            const bool esc = ((am & ~live) >> __lane_id()) & 1u;
+            uint64_t live = am, escm = 0;
+            if (rem >= chunk)
+                fp64_run_formula<false>(f0, s, wz, live, escm, depth, chunk);
+            else
+                fp64_run_formula<true>(f0, s, wz, live, escm, depth, chunk);
+            const bool esc = (escm >> __lane_id()) & 1u;
             if (active && (esc || wz >= depth)) {
                 put_count(out, esc ? (uint32_t)wz : 0u);
                 active = false;

@@ -103,10 +103,9 @@ const char* mdzg_strerror(int code)
    loops are cheap enough per iteration that a refill round every iteration
    costs more than the lanes a longer chunk leaves idle; wider words refill
    every iteration.  The FP64 loop leaves a run early once all its lanes have
-   escaped, and a launch whose pixels all fit its lanes at once (nothing to
-   refill) runs twice the chunk between rounds: 240x180 frame 54.6 us at 512
-   against 58.4 us at 256, while the 3840x2160 frame takes 1.56 ms at 256 and
-   1.92 ms at 512. */
+   escaped; 256 iterations between rounds took the 3840x2160 config 1 frame
+   1.35 ms against 1.41 ms at 128 and 512, with the 240x180 frame and an
+   all-interior frame within 2 % of their best. */
 static int default_chunk(int kid)
 {
     if (kid == 0)
@@ -516,8 +515,6 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
                 uint64_t need = (total + 255) / 256;
                 int grid = (int)(need < cap ? need : cap);
                 int chunk = c->chunk[kid];
-                if (kid == 0 && total <= (uint64_t)grid * 256u)
-                    chunk *= 2;
                 if (mdzg_dev_launch_persist(kid, d_desc + first, cnt, total, d_ctr + k, d_cancel,
                                             grid, chunk, st) != 0) {
                     rc = MDZG_ERR_DEVICE;
