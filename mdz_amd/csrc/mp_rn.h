@@ -118,6 +118,21 @@ MP_HD uint32_t add_cc(uint32_t a, uint32_t b, uint32_t cin, uint32_t& cout)
 #endif
 }
 
+/* a - b - bin, borrow to bout (v_sub_co_u32 / v_subb_co_u32) */
+MP_HD uint32_t sub_bb(uint32_t a, uint32_t b, uint32_t bin, uint32_t& bout)
+{
+#if defined(__clang__)
+    unsigned int bo;
+    const uint32_t r = __builtin_subc(a, b, bin, &bo);
+    bout = bo;
+    return r;
+#else
+    const uint64_t t = (uint64_t)a - b - bin;
+    bout = (uint32_t)(t >> 63);
+    return (uint32_t)t;
+#endif
+}
+
 /* number of power-of-two stages of a shifter covering shifts 0..n */
 constexpr int stages(int n)
 {
@@ -538,10 +553,17 @@ MP_HD void mul_rn(mp<N>& r, const mp<N>& a, const mp<N>& b, int kclr)
 
 /* ------------------------------------------------------------------ */
 /* r = RN_p(a + (-1)^flipb * b)                                         */
+/* KIND ADD_ANY: any signs.  KIND SUB_NONNEG: a - b for a, b >= 0 (the  */
+/* squares' difference of the escape loop): a subtraction of magnitudes */
+/* whatever the swap, so the carry chain is a plain borrow chain and    */
+/* the same-sign carry step cannot occur.  flipb is ignored there.      */
 /* ------------------------------------------------------------------ */
-template <int N>
+enum { ADD_ANY = 0, SUB_NONNEG = 1 };
+
+template <int N, int KIND = ADD_ANY>
 MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb, int kclr)
 {
+    constexpr bool SUBNN = KIND == SUB_NONNEG;
     /* order by exponent: A gets the larger one.  Lanes of a wave walk
        neighbouring pixels whose orbits mostly agree, so a step that no lane
        needs (here: any swap at all) is skipped for the whole wave. */
@@ -559,9 +581,10 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
             B[j] = sel_mask(swm, b_in.m[j], a_in.m[j]);
         }
     }
-    const uint32_t sbin = b_in.s ^ flipb;
-    const uint32_t sa = sel_mask(swm, a_in.s, sbin);
-    const uint32_t sb = sel_mask(swm, sbin, a_in.s);
+    /* signs of A and of the term B: for SUB_NONNEG, a is + and b enters as - */
+    const uint32_t sbin = SUBNN ? 1u : (b_in.s ^ flipb);
+    const uint32_t sa = SUBNN ? sel_mask(swm, 0u, 1u) : sel_mask(swm, a_in.s, sbin);
+    const uint32_t sb = SUBNN ? (sa ^ 1u) : sel_mask(swm, sbin, a_in.s);
     int32_t e = (int32_t)sel_mask(swm, (uint32_t)a_in.e, (uint32_t)b_in.e);
     const int32_t eb = (int32_t)sel_mask(swm, (uint32_t)b_in.e, (uint32_t)a_in.e);
 
@@ -594,18 +617,28 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
     }
 
     /* T = A*2^32 +/- (X + S): one carry chain, X complemented for a
-       subtraction with carry-in 1 - S (A - X - S = A + ~X + 1 - S). */
+       subtraction with carry-in 1 - S (A - X - S = A + ~X + 1 - S); for
+       SUB_NONNEG the borrow chain A*2^32 - X - S itself. */
     const uint32_t sub = sa ^ sb;
-    const uint32_t msk = 0u - sub;
     uint32_t T[N + 1];
-    uint32_t c = sub & (S ^ 1u);
-    T[0] = add_cc(X[0] ^ msk, 0u, c, c);
+    uint32_t c;
+    if (SUBNN) {
+        uint32_t bw;
+        T[0] = sub_bb(0u, X[0], S, bw);
 #pragma unroll
-    for (int j = 1; j <= N; ++j) T[j] = add_cc(A[j - 1], X[j] ^ msk, c, c);
+        for (int j = 1; j <= N; ++j) T[j] = sub_bb(A[j - 1], X[j], bw, bw);
+        c = bw ^ 1u;        /* the carry out of the complemented form */
+    } else {
+        const uint32_t msk = 0u - sub;
+        c = sub & (S ^ 1u);
+        T[0] = add_cc(X[0] ^ msk, 0u, c, c);
+#pragma unroll
+        for (int j = 1; j <= N; ++j) T[j] = add_cc(A[j - 1], X[j] ^ msk, c, c);
+    }
 
     /* same signs with a carry out: shift right one bit */
-    const bool rs = (sub == 0) & (c != 0);
-    if (wave_any(rs)) {
+    const bool rs = !SUBNN && (sub == 0) & (c != 0);
+    if (!SUBNN && wave_any(rs)) {
         S |= rs ? (T[0] & 1u) : 0u;
 #pragma unroll
         for (int j = 0; j < N; ++j) T[j] = rs ? funnel_r(T[j + 1], T[j], 1u) : T[j];
@@ -615,7 +648,7 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
 
     /* a borrow (only possible when d == 0, so S == 0): negate, flip sign */
     uint32_t sres = sa;
-    const bool ng = (sub != 0) & (c == 0);
+    const bool ng = (SUBNN || sub != 0) & (c == 0);
     if (wave_any(ng)) {
         uint32_t cc = ng ? 1u : 0u;
         const uint32_t nm = ng ? 0xffffffffu : 0u;

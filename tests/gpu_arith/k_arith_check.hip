@@ -7,7 +7,8 @@
 //                   const uint32_t* a, const uint32_t* b, uint32_t* r)
 //
 // a, b, r: n stored values of MDZG_VS(nwords) words each (significand words,
-// exponent, sign), host memory.  op: 0 a+b, 1 a-b, 2 a*b, 3 a*a.
+// exponent, sign), host memory.  op: 0 a+b, 1 a-b, 2 a*b, 3 a*a, 4 a-b for
+// nonnegative a and b (the SUB_NONNEG form).
 // Returns 0, -1 for an unsupported word count or bad arguments, -2 on a HIP
 // error.
 #include <hip/hip_runtime.h>
@@ -41,8 +42,10 @@ __global__ __launch_bounds__(256) void k_check(int op, int kclr, int n, const ui
         add_rn<N>(z, x, y, 1u, kclr);
     else if (op == 2)
         mul_rn<N, false>(z, x, y, kclr);
-    else
+    else if (op == 3)
         mul_rn<N, true>(z, x, x, kclr);
+    else
+        add_rn<N, SUB_NONNEG>(z, x, y, 1u, kclr);
 #pragma unroll
     for (int j = 0; j < N; ++j) r[(size_t)i * VS + j] = z.m[j];
     r[(size_t)i * VS + N] = (uint32_t)z.e;
@@ -76,7 +79,7 @@ static int run(int op, int kclr, int n, const uint32_t* a, const uint32_t* b, ui
 extern "C" int arith_check(int nwords, int op, int kclr, int n, const uint32_t* a,
                            const uint32_t* b, uint32_t* r)
 {
-    if (n < 1 || op < 0 || op > 3 || !a || !b || !r)
+    if (n < 1 || op < 0 || op > 4 || !a || !b || !r)
         return -1;
     switch (nwords) {
     case 1: return run<1>(op, kclr, n, a, b, r);

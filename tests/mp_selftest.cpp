@@ -167,11 +167,24 @@ static void check_p(int p, long cases)
         }
         to_mpfr(A, a);
         to_mpfr(B, b);
-        for (int op = 0; op < 4; ++op) {
+        for (int op = 0; op < 5; ++op) {
             if (op == 0) { add_rn<N>(r, a, b, 0u, kclr); mpfr_add(R, A, B, MPFR_RNDN); }
             else if (op == 1) { add_rn<N>(r, a, b, 1u, kclr); mpfr_sub(R, A, B, MPFR_RNDN); }
             else if (op == 2) { mul_rn<N, false>(r, a, b, kclr); mpfr_mul(R, A, B, MPFR_RNDN); }
-            else { mul_rn<N, true>(r, a, a, kclr); mpfr_mul(R, A, A, MPFR_RNDN); }
+            else if (op == 3) { mul_rn<N, true>(r, a, a, kclr); mpfr_mul(R, A, A, MPFR_RNDN); }
+            else {
+                /* |a| - |b| through the specialisation for nonnegative operands */
+                mp<N> pa = a, pb = b;
+                pa.s = 0;
+                pb.s = 0;
+                add_rn<N, SUB_NONNEG>(r, pa, pb, 1u, kclr);
+                mpfr_t PA, PB;
+                mpfr_inits2(32 * N, PA, PB, (mpfr_ptr)0);
+                mpfr_abs(PA, A, MPFR_RNDN);
+                mpfr_abs(PB, B, MPFR_RNDN);
+                mpfr_sub(R, PA, PB, MPFR_RNDN);
+                mpfr_clears(PA, PB, (mpfr_ptr)0);
+            }
             from_mpfr(ref, R);
             if (!same(r, ref)) {
                 if (fails < 10) {
@@ -231,7 +244,7 @@ static void check_chain(int p, long cases)
             mul_rn<N, false>(t, zr, zi, kclr);
             twice(t);
             add_rn<N>(zi, t, ci, 0u, kclr);
-            add_rn<N>(t, zr2, zi2, 1u, kclr);
+            add_rn<N, SUB_NONNEG>(t, zr2, zi2, 1u, kclr);
             add_rn<N>(zr, t, cr, 0u, kclr);
             mul_rn<N, true>(zi2, zi, zi, kclr);
             mul_rn<N, true>(zr2, zr, zr, kclr);

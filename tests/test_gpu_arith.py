@@ -109,15 +109,19 @@ def test_ops_match_mpfr_on_device(n):
             b.append(y)
         A, B = _store(a, n), _store(b, n)
         u32p = ctypes.POINTER(ctypes.c_uint32)
-        for op in range(4):
+        for op in range(5):
+            # op 4: |a| - |b| through the specialisation for nonnegative operands
+            xa = [(M, e, 0) for M, e, _ in a] if op == 4 else a
+            xb = [(M, e, 0) for M, e, _ in b] if op == 4 else b
+            XA, XB = (_store(xa, n), _store(xb, n)) if op == 4 else (A, B)
             R = np.zeros_like(A)
-            rc = L.arith_check(n, op, 32 * n - p, count, A.ctypes.data_as(u32p),
-                               B.ctypes.data_as(u32p), R.ctypes.data_as(u32p))
+            rc = L.arith_check(n, op, 32 * n - p, count, XA.ctypes.data_as(u32p),
+                               XB.ctypes.data_as(u32p), R.ctypes.data_as(u32p))
             assert rc == 0, rc
             for i in range(count):
-                sa = _to_str(a[i], n)
-                sb = _to_str(a[i] if op == 3 else b[i], n)
-                want = rn_ref.parse(O.rn_op((0, 1, 2, 2)[op], p, sa, sb))
+                sa = _to_str(xa[i], n)
+                sb = _to_str(xa[i] if op == 3 else xb[i], n)
+                want = rn_ref.parse(O.rn_op((0, 1, 2, 2, 1)[op], p, sa, sb))
                 got = _value(R[i], n)
                 assert got == want, (n, p, op, i, sa, sb)
 
