@@ -720,26 +720,26 @@ MP_HD bool gt4(const mp<N>& a)
     return (a.s == 0) & !is_zero(a) & ((a.e > 3) | ((a.e == 3) & above));
 }
 
-/* value of a nonnegative a from its top 64 bits (relative error < 2^-52) */
+/* value of a nonnegative a from its top word: a * (1 - 2^-31) < approx <= a */
 template <int N>
 MP_HD double approx_nonneg(const mp<N>& a)
 {
-    const uint32_t hi = a.m[N - 1];
-    const uint32_t lo = (N >= 2) ? a.m[N - 2] : 0u;
-    const double v = (double)hi * 4294967296.0 + (double)lo;
-    return ldexp(v, a.e - 64);
+    return ldexp((double)a.m[N - 1], a.e - 32);
 }
 
 /* RN_p(wim2 + wre2) > 4 for nonnegative wim2, wre2: decided from the top
-   words unless the sum lies within 2^-40 of the threshold 4 + 2^(2-p), in
-   which case the exact sum is rounded (src/frac_mandel.c:47-48). */
+   words unless the sum lies within 2^-26 of the threshold 4 + 2^(2-p), in
+   which case the exact sum is rounded (src/frac_mandel.c:47-48).  Outside
+   that window the top words decide: they fall short of the sum by less than
+   2^-31 of it (under 2^-28 for sums below 8), and the double additions add
+   less than 2^-50. */
 template <int N>
 MP_HD bool escaped(const mp<N>& wim2, const mp<N>& wre2, double thresh, int kclr)
 {
     const double sa = approx_nonneg<N>(wim2) + approx_nonneg<N>(wre2);
     const double dlt = sa - thresh;
     bool esc = dlt > 0.0;
-    const bool unsure = fabs(dlt) <= 0x1p-40;
+    const bool unsure = fabs(dlt) <= 0x1p-26;
     if (wave_any(unsure)) {
         mp<N> t;
         add_rn<N>(t, wim2, wre2, 0u, kclr);
