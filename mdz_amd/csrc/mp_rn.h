@@ -181,12 +181,41 @@ MP_HD void add_ulp(uint32_t* R, int32_t& e, uint32_t inc_word0)
     }
 }
 
+/* R += 1 on the lanes where inc holds, carries ripple upward as in add_ulp.
+   On the device the compare's lane mask is the carry-in of one
+   v_addc_co_u32, and its carry-out mask decides the rare ripple. */
+template <int N>
+MP_HD void add_one_if(uint32_t* R, int32_t& e, bool inc)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint64_t m = __builtin_amdgcn_ballot_w64(inc);
+    uint64_t co;
+    uint32_t r0;
+    asm("v_addc_co_u32 %0, %1, 0, %2, %3" : "=v"(r0), "=s"(co) : "v"(R[0]), "s"(m));
+    const uint32_t old = R[0];
+    R[0] = r0;
+    if (co != 0) {                    /* R[0] wrapped on some lane: rare */
+        uint32_t c = (uint32_t)(r0 < old);
+#pragma unroll
+        for (int j = 1; j < N; ++j) R[j] = add_cc(R[j], 0u, c, c);
+        if (c) {
+            R[N - 1] = 0x80000000u;
+            e += 1;
+        }
+    }
+#else
+    add_ulp<N>(R, e, inc ? 1u : 0u);
+#endif
+}
+
+/* S is 0 or 1.  Nearest-even rounds up when G exceeds the half 2^31, or
+   equals it with a sticky bit or an odd last bit: with x = S | lsb (0 or 1)
+   that is G > 2^31 - x. */
 template <int N>
 MP_HD void round_full(uint32_t* R, uint32_t G, uint32_t S, int32_t& e)
 {
-    const uint32_t below = (G << 1) | S;
-    const uint32_t inc = (G >> 31) & (uint32_t)((below != 0) | (R[0] & 1u));
-    add_ulp<N>(R, e, inc);
+    const uint32_t x = (R[0] & 1u) | S;
+    add_one_if<N>(R, e, G > 0x80000000u - x);
 }
 
 /* true when the predicate holds on every lane of the wave (device) / on
