@@ -295,10 +295,14 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
     bool drained = false;
     wave_run run = { 0, 0u };
     const uint32_t run_len = share_run(total);
+    /* a launch with no more pixels than lanes (the 240x180 frame) takes them
+       by lane index: no queue atomic to start, none to find the queue empty */
+    const bool by_lane = total <= (uint64_t)gridDim.x * blockDim.x;
     for (;;) {
         if (!drained) {
             const bool need = !active;
-            const uint64_t g = claim_len(need, counter, run, run_len);
+            const uint64_t g = by_lane ? (uint64_t)blockIdx.x * blockDim.x + threadIdx.x
+                                       : claim_len(need, counter, run, run_len);
             if (need && g < total) {
                 const int ti = find_tile(tiles, ntiles, g);
                 const mdzg_dtile& t = tiles[ti];
@@ -318,7 +322,7 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
             }
             /* the counter only grows: once one lane reads past the end the wave
                stops claiming, so no atomics while its last lanes finish */
-            drained = __any(need && g >= total);
+            drained = by_lane || __any(need && g >= total);
         }
         if (!__any(active))
             break;
