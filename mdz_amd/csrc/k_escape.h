@@ -610,6 +610,16 @@ __global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_persist(c
     s.c.init(c_lds);
     bool drained = false;
     wave_run run = { 0, 0u };
+    /* no more pixels than lanes: each wave starts with its own 64 slots as
+       its run (taken without an atomic) and never claims again (config 2:
+       +2.5 %).  Not from 24 words: the start-up code changed the register
+       allocation of those 256-VGPR kernels, 6 % slower on the 1024-bit band,
+       whose launches never fit their lanes. */
+    const bool by_lane = N < 24 && total <= (uint64_t)gridDim.x * blockDim.x;
+    if (by_lane) {
+        run.base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+        run.left = 64u;
+    }
     for (;;) {
         if (!drained) {
             const bool need = !active;
@@ -629,7 +639,7 @@ __global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_persist(c
                 active = true;
                 asm volatile("" ::"v"(fractal), "v"(kclr), "v"(depth), "v"(out));
             }
-            drained = __any(need && g >= total);
+            drained = by_lane || __any(need && g >= total);
         }
         if (!__any(active))
             break;

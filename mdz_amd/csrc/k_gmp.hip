@@ -119,6 +119,13 @@ __global__ __launch_bounds__(256) void k_gmp_persist(const mdzg_dtile* tiles, in
     gmp_state<L> s;
     bool drained = false;
     wave_run run = { 0, 0u };
+    /* no more pixels than lanes: each wave starts with its own 64 slots as
+       its run (taken without an atomic) and never claims again */
+    const bool by_lane = total <= (uint64_t)gridDim.x * blockDim.x;
+    if (by_lane) {
+        run.base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+        run.left = 64u;
+    }
     for (;;) {
         if (!drained) {
             const bool need = !active;
@@ -140,7 +147,7 @@ __global__ __launch_bounds__(256) void k_gmp_persist(const mdzg_dtile* tiles, in
                    (there the wait would include the count stores in flight) */
                 asm volatile("" ::"v"(fractal), "v"(lr), "v"(depth), "v"(out));
             }
-            drained = __any(need && g >= total);
+            drained = by_lane || __any(need && g >= total);
         }
         if (!__any(active))
             break;
