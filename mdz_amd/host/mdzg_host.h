@@ -81,6 +81,10 @@ int mdzg_class_of(int path, long precision, int* nw, int* kclr);
 int mdzg_coords_fill(const mdzg_pview* pv, int line0, int nlines, int col0, int ncols,
                      uint32_t* xs, uint32_t* ys, uint32_t* jc);
 
+/* dst = src (n counts), returning their pixel-iterations (zero counts
+   standing for depth) */
+double mdzg_take_counts(uint32_t* dst, const uint32_t* src, size_t n, double depth);
+
 /* FP64 descriptor fields (xmin, ymax, width, Julia c) */
 void mdzg_coords_fp64(const mdzg_pview* pv, double* xmin, double* ymax, double* width,
                       double* jre, double* jim);

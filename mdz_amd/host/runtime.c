@@ -640,6 +640,23 @@ int mdzg_cancel(mdzg_job* j)
     return MDZG_OK;
 }
 
+/* dst = src (n counts) and their pixel-iterations, a zero count standing for
+   the depth: one pass with integer sums (exact, like the double sum of each
+   term, below 2^53), which the compiler vectorises.  A double accumulator
+   carried one 4-cycle add per pixel: 0.3 ms of host time per config 2 frame. */
+__attribute__((optimize("O3"))) double mdzg_take_counts(uint32_t* dst, const uint32_t* src,
+                                                        size_t n, double depth)
+{
+    uint64_t sum = 0, zeros = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const uint32_t v = src[k];
+        dst[k] = v;
+        sum += v;
+        zeros += (v == 0);
+    }
+    return (double)sum + (double)zeros * depth;
+}
+
 int mdzg_wait(mdzg_job* j, mdzg_stats* st)
 {
     if (!j)
@@ -662,9 +679,7 @@ int mdzg_wait(mdzg_job* j, mdzg_stats* st)
             size_t n = (size_t)t->nlines * (size_t)t->ncols;
             const double depth = (double)t->view.depth;
             if (!j->cancelled) {
-                memcpy(t->counts, src, n * 4);
-                for (size_t k = 0; k < n; ++k)
-                    pi += src[k] ? (double)src[k] : depth;
+                pi += mdzg_take_counts(t->counts, src, n, depth);
             } else {
                 /* keep the caller's contents where the device never wrote */
                 for (size_t k = 0; k < n; ++k)

@@ -209,12 +209,16 @@ static void fill_info(const sreq* r, mdzg_req_info* info)
     info->pixel_iters = r->pixel_iters;
 }
 
+/* pixel-iterations of n counts (a zero count stands for the depth), summed
+   in integers as mdzg_take_counts does */
 static double count_iters(const uint32_t* c, size_t n, double depth)
 {
-    double pi = 0.0;
-    for (size_t k = 0; k < n; ++k)
-        pi += c[k] ? (double)c[k] : depth;
-    return pi;
+    uint64_t sum = 0, zeros = 0;
+    for (size_t k = 0; k < n; ++k) {
+        sum += c[k];
+        zeros += (c[k] == 0);
+    }
+    return (double)sum + (double)zeros * depth;
 }
 
 /* ---- batched mode: one slice ------------------------------------------------ */
