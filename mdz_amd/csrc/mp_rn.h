@@ -538,13 +538,13 @@ MP_HD void mant_product(const uint32_t* a, const uint32_t* b, uint32_t* H, uint3
 template <int N>
 MP_HD void norm_product(uint32_t* H, int32_t& e)
 {
-    const bool sh = (H[N] >> 31) == 0;
-    if (wave_any(sh)) {
+    const uint32_t dec = (H[N] >> 31) ^ 1u;       /* 1 where a shift is needed */
+    if (wave_any(dec != 0)) {
 #pragma unroll
         for (int j = N; j >= 1; --j)
-            H[j] = sh ? funnel_r(H[j], H[j - 1], 31u) : H[j];
-        H[0] = sh ? (H[0] << 1) : H[0];
-        e -= sh ? 1 : 0;
+            H[j] = dec ? funnel_r(H[j], H[j - 1], 31u) : H[j];
+        H[0] = dec ? (H[0] << 1) : H[0];
+        e -= (int32_t)dec;
     }
 }
 
@@ -677,13 +677,14 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
 
     /* a borrow (only possible when d == 0, so S == 0): negate, flip sign */
     uint32_t sres = sa;
-    const bool ng = (SUBNN || sub != 0) & (c == 0);
-    if (wave_any(ng)) {
-        uint32_t cc = ng ? 1u : 0u;
-        const uint32_t nm = ng ? 0xffffffffu : 0u;
+    /* 0/1 from the data (c is the carry out, 0 or 1), used as a number */
+    const uint32_t ngv = (SUBNN ? 1u : sub) & (c ^ 1u);
+    if (wave_any(ngv != 0)) {
+        uint32_t cc = ngv;
+        const uint32_t nm = 0u - ngv;
 #pragma unroll
         for (int j = 0; j <= N; ++j) T[j] = add_cc(T[j] ^ nm, 0u, cc, cc);
-        sres = ng ? (sres ^ 1u) : sres;
+        sres ^= ngv;
     }
 
     /* cancellation of the whole top word (a subtraction with d <= 1, which
