@@ -334,8 +334,9 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
            depth within it (the wave minimum of the remaining depths), the
            run skips the per-lane depth test.  Cutting the run at that
            minimum instead made the runs of waves whose pixels started at
-           different times short: 1.60 against 1.82 T pixel-iter/s on the
-           3840x2160 config 1 frame. */
+           different times short: 1.60 against 1.82 T pixel-iter/s on a
+           3840x2160 frame of the whole set (centre -0.75, width 3, depth
+           1000). */
         const uint64_t am = __ballot(active);
         const int lead = __ffsll((long long)am) - 1;
         const int f0 = __shfl(fractal, lead);
@@ -611,8 +612,8 @@ __global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_persist(c
     bool drained = false;
     wave_run run = { 0, 0u };
     /* no more pixels than lanes: each wave starts with its own 64 slots as
-       its run (taken without an atomic) and never claims again (config 2:
-       +2.5 %).  Not from 24 words: the start-up code changed the register
+       its run (taken without an atomic) and never claims again (a 640x480
+       frame at 128 bits: +2.5 %).  Not from 24 words: the start-up code changed the register
        allocation of those 256-VGPR kernels, 6 % slower on the 1024-bit band,
        whose launches never fit their lanes. */
     const bool by_lane = N < 24 && total <= (uint64_t)gridDim.x * blockDim.x;

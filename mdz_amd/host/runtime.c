@@ -99,11 +99,11 @@ const char* mdzg_strerror(int code)
 }
 
 /* iterations a lane runs between queue refills, by kernel class (measured on
-   the config 1 view at 240x180 and 3840x2160): the FP64 and 32/64-bit RN
-   loops are cheap enough per iteration that a refill round every iteration
-   costs more than the lanes a longer chunk leaves idle; wider words refill
-   every iteration.  The FP64 loop leaves a run early once all its lanes have
-   escaped; 256 iterations between rounds took the 3840x2160 config 1 frame
+   the whole set, centre -0.75, width 3, at 240x180 and 3840x2160): the FP64
+   and 32/64-bit RN loops are cheap enough per iteration that a refill round
+   every iteration costs more than the lanes a longer chunk leaves idle; wider
+   words refill every iteration.  The FP64 loop leaves a run early once all its lanes have
+   escaped; 256 iterations between rounds took the 3840x2160 frame
    1.35 ms against 1.41 ms at 128 and 512, with the 240x180 frame and an
    all-interior frame within 2 % of their best. */
 static int default_chunk(int kid)
@@ -643,7 +643,7 @@ int mdzg_cancel(mdzg_job* j)
 /* dst = src (n counts) and their pixel-iterations, a zero count standing for
    the depth: one pass with integer sums (exact, like the double sum of each
    term, below 2^53), which the compiler vectorises.  A double accumulator
-   carried one 4-cycle add per pixel: 0.3 ms of host time per config 2 frame. */
+   carried one 4-cycle add per pixel: 0.3 ms of host time per 640x480 frame. */
 __attribute__((optimize("O3"))) double mdzg_take_counts(uint32_t* dst, const uint32_t* src,
                                                         size_t n, double depth)
 {
