@@ -766,6 +766,11 @@ MP_HD double approx_nonneg(const mp<N>& a)
 template <int N>
 MP_HD bool escaped(const mp<N>& wim2, const mp<N>& wre2, double thresh, int kclr)
 {
+    /* both squares below 2 (exponent <= 1) on every lane of the wave: sums
+       below 4, no escape, and no estimate to form (most iterations of a bounded
+       orbit take this exit) */
+    if (wave_all((wim2.e <= 1) & (wre2.e <= 1)))
+        return false;
     const double sa = approx_nonneg<N>(wim2) + approx_nonneg<N>(wre2);
     const double dlt = sa - thresh;
     bool esc = dlt > 0.0;

@@ -101,17 +101,20 @@ const char* mdzg_strerror(int code)
 /* iterations a lane runs between queue refills, by kernel class (measured on
    the whole set, centre -0.75, width 3, at 240x180 and 3840x2160): the FP64
    and 32/64-bit RN loops are cheap enough per iteration that a refill round
-   every iteration costs more than the lanes a longer chunk leaves idle; wider
-   words refill every iteration.  The FP64 loop leaves a run early once all its lanes have
-   escaped; 256 iterations between rounds took the 3840x2160 frame
-   1.35 ms against 1.41 ms at 128 and 512, with the 240x180 frame and an
-   all-interior frame within 2 % of their best. */
+   every iteration costs more than the lanes a longer chunk leaves idle.  Wider
+   words run 8 (a 640x480 frame at 128 bits +3.7 % against 1, a 1024-bit band
+   +1.6 % at 4): a few idle iterations per finished pixel cost less than a
+   round per iteration.  The GMP loops refill every iteration.  The FP64 loop
+   leaves a run early once all its lanes have escaped; 256 iterations between
+   rounds took the 3840x2160 frame 1.35 ms against 1.41 ms at 128 and 512,
+   with the 240x180 frame and an all-interior frame within 2 % of their
+   best. */
 static int default_chunk(int kid)
 {
     if (kid == 0)
         return 256;
-    if (kid >= 1 && kid < MDZG_KID_GMP0 && mdzg_nwords[(kid - 1) >> 1] <= 2)
-        return 16;
+    if (kid >= 1 && kid < MDZG_KID_GMP0)
+        return mdzg_nwords[(kid - 1) >> 1] <= 2 ? 16 : 8;
     return 1;
 }
 
