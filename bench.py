@@ -41,7 +41,7 @@ COLOUR_TRAFFIC_BYTES = {1: 66.4e6, 2: 166.0e6}
 CONFIG2_TRAFFIC_BYTES = 3.07e6
 # VALU instructions (wave64) per config 2 launch of k_rn_persist<4,true>:
 # rocprofv3 SQ_INSTS_VALU of the current kernel build
-CONFIG2_VALU_INSTS = 8.0015e9
+CONFIG2_VALU_INSTS = 7.6552e9
 # VALU issue limit: 256 CUs x one wave64 instruction per clock x 2.4 GHz peak clock
 PEAK_VALU_ISSUE = 256 * 2.4e9
 # VALU instructions per wave-iteration of the FP64 kernel's run loop: the
