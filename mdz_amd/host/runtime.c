@@ -104,7 +104,7 @@ const char* mdzg_strerror(int code)
    every iteration costs more than the lanes a longer chunk leaves idle.  Wider
    words run 8 (a 640x480 frame at 128 bits +3.7 % against 1, a 1024-bit band
    +1.6 % at 4): a few idle iterations per finished pixel cost less than a
-   round per iteration.  The GMP loops refill every iteration.  The FP64 loop
+   round per iteration; the GMP loops too (+1.3 % at 128 bits).  The FP64 loop
    leaves a run early once all its lanes have escaped; 256 iterations between
    rounds took the 3840x2160 frame 1.35 ms against 1.41 ms at 128 and 512,
    with the 240x180 frame and an all-interior frame within 2 % of their
@@ -115,7 +115,7 @@ static int default_chunk(int kid)
         return 256;
     if (kid >= 1 && kid < MDZG_KID_GMP0)
         return mdzg_nwords[(kid - 1) >> 1] <= 2 ? 16 : 8;
-    return 1;
+    return 8;
 }
 
 int mdzg_create(int device, mdzg_ctx** out)
