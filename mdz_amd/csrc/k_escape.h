@@ -202,11 +202,14 @@ __global__ __launch_bounds__(256) void k_fp64_tile(const mdzg_dtile* tp)
     fp64_state s;
     fp64_init(s, t, row, col);
     uint32_t count = 0;
-    for (int32_t wz = 1; wz <= t.depth; ++wz) {
+    /* the exit test follows the step, so depth == INT32_MAX ends too */
+    for (int32_t wz = 1;; ++wz) {
         if (fp64_step(s, t.fractal, wz)) {
             count = (uint32_t)wz;
             break;
         }
+        if (wz >= t.depth)
+            break;
     }
     t.out[idx] = count;
 }
@@ -586,11 +589,13 @@ __global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_tile(cons
     const int fractal = t.fractal, kclr = t.kclr;
     const int32_t depth = t.depth;
     uint32_t count = 0;
-    for (int32_t wz = 1; wz <= depth; ++wz) {
+    for (int32_t wz = 1;; ++wz) {
         if (rn_step<N, FULL>(s, fractal, wz, kclr)) {
             count = (uint32_t)wz;
             break;
         }
+        if (wz >= depth)
+            break;
     }
     t.out[idx] = count;
 }

@@ -97,11 +97,13 @@ __global__ __launch_bounds__(256) void k_gmp_tile(const mdzg_dtile* tp)
     const int fractal = t.fractal, lr = t.gprec;
     const int32_t depth = t.depth;
     uint32_t count = 0;
-    for (int32_t wz = 1; wz <= depth; ++wz) {
+    for (int32_t wz = 1;; ++wz) {   /* ends at depth == INT32_MAX too */
         if (gmp_step<L>(s, fractal, wz, lr)) {
             count = (uint32_t)wz;
             break;
         }
+        if (wz >= depth)
+            break;
     }
     t.out[idx] = count;
 }
