@@ -214,6 +214,27 @@ int mdzg_scene_rect(const mdzg_scene* s, int user_w, int user_h, int flags,
    those rows and columns (src/fractal.c:29-257), in one persistent launch. */
 int mdzg_render_tile(mdzg_ctx* ctx, const mdzg_tile* tile, mdzg_stats* stats);
 
+/* ---- render without allocating (no counterpart in the reference) --------
+   For callers that cannot allocate device memory: code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc.  The same
+   render as mdzg_render_tile, except that tile->counts is DEVICE memory of
+   the context's device (nlines * ncols uint32, written in place) and every
+   other device buffer of the render -- the tile descriptor, the queue
+   counter, the cancel flag and the start values -- lives in a static device
+   array of MDZG_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.
+   The call makes no hipMalloc, hipHostMalloc or hipFree; it is synchronous
+   on the context's stream.  MDZG_ERR_RANGE when the tile's start values do
+   not fit the array: (ncols + nlines + 2) values of 16 * ceil((N + 2) / 4)
+   bytes with N = ceil(p / 32) words (MPFR and long double paths) or
+   16 * ceil((2L + 3) / 4) bytes with L the GMP limb bucket; FP64 tiles need
+   none.  MDZG_ERR_ARG when tile->counts is not device memory of the
+   context's device.  Every call on a GPU shares that GPU's array, which
+   keeps the last call's contents: calls are serialised within a process
+   (calls that overlap wait for each other).  stats->pixel_iters is 0: the
+   counts are not read back. */
+#define MDZG_NOALLOC_SCRATCH_BYTES (8u << 20)
+int mdzg_render_tile_noalloc(mdzg_ctx* ctx, const mdzg_tile* tile, mdzg_stats* stats);
+
 /* Batch modes */
 enum { MDZG_BATCH_PERSISTENT = 0,   /* one persistent launch per arithmetic class,
                                        draining a device queue of all tiles' pixels */

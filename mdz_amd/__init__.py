@@ -145,6 +145,7 @@ def lib():
         L.mdzg_scene_rect.argtypes = [P(Scene), ctypes.c_int, ctypes.c_int, ctypes.c_int] + \
             [ctypes.c_char_p] * 4 + [ctypes.c_int]
         L.mdzg_render_tile.argtypes = [vp, P(Tile), P(Stats)]
+        L.mdzg_render_tile_noalloc.argtypes = [vp, P(Tile), P(Stats)]
         L.mdzg_render_batch.argtypes = [vp, P(Tile), ctypes.c_int, ctypes.c_int, P(Stats)]
         L.mdzg_submit.argtypes = [vp, P(Tile), ctypes.c_int, P(vp)]
         L.mdzg_poll.argtypes = [vp]
@@ -331,6 +332,19 @@ class Context:
         st = Stats()
         _check(lib().mdzg_render_tile(self._c, ctypes.byref(t), ctypes.byref(st)), "render")
         return out, st.as_dict()
+
+    def render_noalloc(self, view, counts_ptr, lines=None, cols=None):
+        """mdzg_render_tile_noalloc: the counts of a rectangle written to device
+        memory at address counts_ptr (nlines * ncols uint32 on this context's
+        device), with no device allocation; returns the stats."""
+        line0, nlines = lines if lines is not None else (0, view.real_height)
+        col0, ncols = cols if cols is not None else (0, view.real_width)
+        t = Tile(view, line0, nlines, col0, ncols,
+                 ctypes.cast(ctypes.c_void_p(counts_ptr), ctypes.POINTER(ctypes.c_uint32)))
+        st = Stats()
+        _check(lib().mdzg_render_tile_noalloc(self._c, ctypes.byref(t), ctypes.byref(st)),
+               "render_noalloc")
+        return st.as_dict()
 
     def colour(self, counts, aa, palette, stats=False):
         """rgb_data [h, w] uint32 of a [h*aa, w*aa] count array (mdzg_colour);

@@ -63,6 +63,8 @@ def jobs():
                    "-o", os.path.join(BUILD, "k_fp64.o")], "k_fp64.o"))
     steps.append(([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "k_colour.hip"),
                    "-o", os.path.join(BUILD, "k_colour.o")], "k_colour.o"))
+    steps.append(([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "noalloc.hip"),
+                   "-o", os.path.join(BUILD, "noalloc.o")], "noalloc.o"))
     for n in sorted(NWORDS, reverse=True):   # largest (slowest) first
         steps.append(([hipcc, *HIP_FLAGS, f"-DMDZG_N={n}", "-c", os.path.join(CSRC, "k_rn.hip"),
                        "-o", os.path.join(BUILD, f"k_rn{n}.o")], f"k_rn{n}.o"))

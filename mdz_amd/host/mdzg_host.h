@@ -107,7 +107,18 @@ struct mdzg_ctx {
     int occ[MDZG_NKERNELS];     /* resident blocks per CU (persistent grid) */
     int chunk[MDZG_NKERNELS];   /* iterations between lane refills */
     pthread_mutex_t lock;
+    /* mdzg_render_tile_noalloc: the device address of the library's static
+       scratch array on this device (noalloc.hip), the host buffer its
+       contents are staged in, and the two events that time its launch, all
+       set up by mdzg_create */
+    void* d_scratch;
+    void* h_scratch;
+    hipEvent_t na_ev0, na_ev1;
 };
+
+/* noalloc.hip: device address of the static scratch array of the current
+   device (MDZG_NOALLOC_SCRATCH_BYTES); 0 on success */
+int mdzg_dev_scratch(int device, void** ptr, size_t* bytes);
 
 double mdzg_now_ms(void);
 

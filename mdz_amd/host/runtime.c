@@ -147,6 +147,14 @@ int mdzg_create(int device, mdzg_ctx** out)
         goto fail;
     if (hipMemset(c->d_cancel_zero, 0, 256) != hipSuccess)
         goto fail;
+    /* the no-allocation entry point's scratch: its device address on this
+       device, a host staging buffer of the same size, two timing events */
+    if (mdzg_dev_scratch(device, &c->d_scratch, NULL) != 0)
+        goto fail;
+    c->h_scratch = malloc(MDZG_NOALLOC_SCRATCH_BYTES);
+    if (!c->h_scratch || hipEventCreate(&c->na_ev0) != hipSuccess ||
+        hipEventCreate(&c->na_ev1) != hipSuccess)
+        goto fail;
     for (int k = 0; k < mdzg_dev_nkernels(); ++k) {
         c->occ[k] = mdzg_dev_occupancy(k, 1);
         if (c->occ[k] < 1)
@@ -158,6 +166,7 @@ int mdzg_create(int device, mdzg_ctx** out)
     *out = c;
     return MDZG_OK;
 fail:
+    free(c->h_scratch);
     free(f);
     return MDZG_ERR_DEVICE;
 }
@@ -178,6 +187,9 @@ void mdzg_destroy(mdzg_ctx* c)
     if (c->stream) hipStreamDestroy(c->stream);
     if (c->ctl) hipStreamDestroy(c->ctl);
     if (c->d_cancel_zero) hipFree(c->d_cancel_zero);
+    if (c->na_ev0) hipEventDestroy(c->na_ev0);
+    if (c->na_ev1) hipEventDestroy(c->na_ev1);
+    free(c->h_scratch);
     pthread_mutex_destroy(&c->lock);
     pthread_mutex_destroy(&f->rt.cache_lock);
     free(f);
@@ -819,4 +831,104 @@ int mdzg_rt_render_image(mdzg_ctx* c, const mdzg_tile* tile, const mdzg_image_re
 int mdzg_render_tile(mdzg_ctx* c, const mdzg_tile* tile, mdzg_stats* st)
 {
     return mdzg_render_batch(c, tile, 1, MDZG_BATCH_PERSISTENT, st);
+}
+
+/* ---- rendering without allocating ---------------------------------------
+   mdzg_render_tile's work for callers that cannot allocate device memory:
+   the counts go straight to the caller's device buffer, and the descriptor,
+   queue counter, cancel flag and start values to the static scratch array
+   of noalloc.hip (its device address c->d_scratch), staged through the
+   context's host buffer in one copy.  Nothing is allocated: the events and
+   the host buffer were made by mdzg_create.  One lock per device serialises
+   the calls of this process, which share that device's array. */
+#define NA_OFF_CTR 256                    /* queue counter (8 B), cancel flag (+8) */
+#define NA_OFF_COORDS 512                 /* start values                          */
+#define NA_MAX_DEVICES 64
+static pthread_mutex_t na_lock[NA_MAX_DEVICES] = { [0 ... NA_MAX_DEVICES - 1] =
+                                                       PTHREAD_MUTEX_INITIALIZER };
+
+int mdzg_render_tile_noalloc(mdzg_ctx* c, const mdzg_tile* t, mdzg_stats* st)
+{
+    const double t0 = mdzg_now_ms();
+    if (!c || !tile_ok(t) || c->device < 0 || c->device >= NA_MAX_DEVICES)
+        return MDZG_ERR_ARG;
+    if (!c->d_scratch || !c->h_scratch)
+        return MDZG_ERR_STATE;
+    hipSetDevice(c->device);
+    /* the counts must be device memory of this device */
+    hipPointerAttribute_t pa;
+    memset(&pa, 0, sizeof pa);
+    if (hipPointerGetAttributes(&pa, t->counts) != hipSuccess ||
+        pa.type != hipMemoryTypeDevice || pa.device != c->device) {
+        (void)hipGetLastError();
+        return MDZG_ERR_ARG;
+    }
+    mdzg_pview pv;
+    int rc = mdzg_view_parse(&t->view, &pv);
+    if (rc != MDZG_OK)
+        return rc;
+    const size_t vsb = 4 * (size_t)pv.vs;
+    const size_t need = NA_OFF_COORDS +
+        (pv.vs ? ALIGN_UP(((size_t)t->ncols + (size_t)t->nlines + 2) * vsb, 16) : 0);
+    if (need > MDZG_NOALLOC_SCRATCH_BYTES) {
+        mdzg_view_clear(&pv);
+        return MDZG_ERR_RANGE;
+    }
+    pthread_mutex_lock(&na_lock[c->device]);
+    char* H = (char*)c->h_scratch;
+    char* D = (char*)c->d_scratch;
+    memset(H, 0, NA_OFF_COORDS);          /* descriptor, counter and cancel flag */
+    mdzg_dtile* d = (mdzg_dtile*)H;
+    d->out = t->counts;
+    d->ncols = t->ncols;
+    d->nrows = t->nlines;
+    d->col0 = t->col0;
+    d->line0 = t->line0;
+    d->W = t->view.real_width;
+    d->fractal = t->view.fractal;
+    d->family = t->view.family;
+    d->kclr = pv.kclr;
+    d->depth = (int32_t)t->view.depth;
+    d->vs = pv.vs;
+    d->gprec = pv.v.path == MDZG_PATH_GMP ? pv.nw : 0;
+    d->pix_begin = 0;
+    if (pv.vs) {
+        uint32_t* xs = (uint32_t*)(H + NA_OFF_COORDS);
+        uint32_t* ys = (uint32_t*)(H + NA_OFF_COORDS + vsb * (size_t)t->ncols);
+        uint32_t* jc = (uint32_t*)(H + NA_OFF_COORDS + vsb * ((size_t)t->ncols + t->nlines));
+        rc = mdzg_coords_fill(&pv, t->line0, t->nlines, t->col0, t->ncols, xs, ys, jc);
+        d->xs = (const uint32_t*)(D + NA_OFF_COORDS);
+        d->ys = (const uint32_t*)(D + NA_OFF_COORDS + vsb * (size_t)t->ncols);
+        d->jc = (const uint32_t*)(D + NA_OFF_COORDS + vsb * ((size_t)t->ncols + t->nlines));
+    } else {
+        mdzg_coords_fp64(&pv, &d->xmin, &d->ymax, &d->width, &d->jre, &d->jim);
+    }
+    const int kid = pv.kid;
+    mdzg_view_clear(&pv);
+    if (rc == MDZG_OK) {
+        const uint64_t total = (uint64_t)t->ncols * (uint64_t)t->nlines;
+        const uint64_t cap = (uint64_t)c->ncu * (uint64_t)c->occ[kid];
+        const uint64_t blocks = (total + 255) / 256;
+        const int grid = (int)(blocks < cap ? blocks : cap);
+        if (hipMemcpyAsync(D, H, need, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipEventRecord(c->na_ev0, c->stream) != hipSuccess ||
+            mdzg_dev_launch_persist(kid, (const mdzg_dtile*)D, 1, total,
+                                    (unsigned long long*)(D + NA_OFF_CTR),
+                                    (const int*)(D + NA_OFF_CTR + 8), grid, c->chunk[kid],
+                                    c->stream) != 0 ||
+            hipEventRecord(c->na_ev1, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            hipStreamSynchronize(c->stream);
+            rc = MDZG_ERR_DEVICE;
+        } else if (st) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, c->na_ev0, c->na_ev1);
+            st->kernel_ms = ms;
+            st->wall_ms = mdzg_now_ms() - t0;
+            st->pixel_iters = 0.0;
+            st->launches = 1;
+        }
+    }
+    pthread_mutex_unlock(&na_lock[c->device]);
+    return rc;
 }
