@@ -318,10 +318,10 @@ class Context:
         self.close()
 
     @staticmethod
-    def _tile(view, lines, cols):
+    def _tile(view, lines, cols, fill=0):
         line0, nlines = lines if lines is not None else (0, view.real_height)
         col0, ncols = cols if cols is not None else (0, view.real_width)
-        out = np.zeros((nlines, ncols), dtype=np.uint32)
+        out = np.full((nlines, ncols), fill, dtype=np.uint32)
         t = Tile(view, line0, nlines, col0, ncols,
                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
         return t, out
@@ -397,18 +397,19 @@ class Context:
     def tune_set(self, kernel_id, blocks_per_cu=0, chunk=0):
         _check(lib().mdzg_tune_set(self._c, kernel_id, blocks_per_cu, chunk), "tune_set")
 
-    def submit(self, specs):
-        """Asynchronous batch: returns a Job."""
-        return Job(self, specs)
+    def submit(self, specs, fill=0):
+        """Asynchronous batch: returns a Job.  The counts buffers start at
+        `fill` (a cancelled job leaves unfinished pixels at that value)."""
+        return Job(self, specs, fill)
 
 
 class Job:
-    def __init__(self, ctx, specs):
+    def __init__(self, ctx, specs, fill=0):
         n = len(specs)
         self._arr = (Tile * n)()
         self.counts = []
         for i, (v, lines, cols) in enumerate(specs):
-            t, out = Context._tile(v, lines, cols)
+            t, out = Context._tile(v, lines, cols, fill)
             self._arr[i] = t
             self.counts.append(out)
         self._views = [s[0] for s in specs]

@@ -33,9 +33,55 @@ def test_config1_full_frame(ctx, oracle, path, fractal):
     assert st["pixel_iters"] == float(np.where(ref > 0, ref, 1000).sum())
 
 
+def _golden_view(vd):
+    return M.make_view(vd["rect"], precision=vd["precision"], depth=vd["depth"],
+                       width=vd["width"], height=vd["height"], fractal=vd["fractal"],
+                       family=vd["family"], path=vd["path"], julia=tuple(vd["julia"]))
+
+
+def _golden_meta():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                           "golden.json")) as f:
+        return json.load(f)["cases"]
+
+
+DEEP = [c["name"] for c in _golden_meta() if c["name"].startswith("deep_")]
+
+
+@pytest.mark.parametrize("name", DEEP)
+def test_deep_golden_map(ctx, golden, name):
+    """Boundary crops at every precision class, formula and path, down to
+    width 1e-290 (p = 1024, depth 20000), whose pixels escape at many
+    different iterations: MDZ's own loops reproduced on the device."""
+    meta, expect = next((m, e) for m, e in golden if m["name"] == name)
+    v = _golden_view(meta["view"])
+    print(name, flush=True)
+    got, st = ctx.render(v)
+    assert len(np.unique(expect)) >= 20
+    assert _eq(got, expect), name
+    d = meta["view"]["depth"]
+    assert st["pixel_iters"] == float(np.where(expect > 0, expect, d).sum())
+
+
+def test_deep_maps_in_one_batch(ctx, golden):
+    """All deep crops as the tiles of one batch (every arithmetic class at
+    once): persistent and per-tile forms both equal the maps."""
+    deep = [(m, e) for m, e in golden if m["name"].startswith("deep_")]
+    specs = [(_golden_view(m["view"]), None, None) for m, _ in deep]
+    for mode in (M.BATCH_PERSISTENT, M.BATCH_PER_TILE):
+        print("deep batch mode", mode, flush=True)
+        outs, _ = ctx.render_batch(specs, mode=mode)
+        for (m, e), got in zip(deep, outs):
+            assert _eq(got, e), (mode, m["name"])
+
+
 def test_golden_maps(ctx, golden):
     """Every committed map (MDZ's own loops) reproduced on the device."""
     for meta, expect in golden:
+        if meta["name"].startswith("deep_"):
+            continue              # test_deep_golden_map
         vd = meta["view"]
         v = M.make_view(vd["rect"], precision=vd["precision"], depth=vd["depth"],
                         width=vd["width"], height=vd["height"], fractal=vd["fractal"],
@@ -331,16 +377,31 @@ def test_frame_stop_keeps_sentinel(ctx):
     assert (raw[-1] == -1).all()
 
 
-def test_async_submit_and_cancel(ctx):
-    v = WL.config2_view()
-    job = ctx.submit([(v, (0, 480), (0, 640))])
+def test_async_submit_and_cancel(ctx, golden):
+    # a 1024-bit boundary view (escapes from ~3000 to 20000 iterations) at
+    # 192 x 192, some 40 ms of device work, cancelled right after submission
+    meta = next(m for m, _ in golden if m["name"] == "deep_mpfr1024_f2")
+    vd = dict(meta["view"], width=192, height=192)
+    cx, cy = meta["view"]["rect"][0], meta["view"]["rect"][2]   # corner of the crop
+    rect = M.rect_from_center(cx, cy, "1e-249", 1024, 192, 192)
+    v = M.make_view(rect, precision=1024, depth=vd["depth"], width=192, height=192,
+                    fractal=2)
+    full, _ = ctx.render(v)
+    assert len(np.unique(full)) > 100
+    SENT = 0xFFFFFFFF
+    job = ctx.submit([(v, None, None)], fill=SENT)
     job.cancel()
     rc, st = job.wait()
     job.free()
     assert rc in (0, -4)
     c = job.counts[0]
-    # finished pixels are exact (0 = never escaped); unfinished keep their zero init
-    assert ((c == 0)).all()
+    done = c != SENT
+    # finished pixels equal the uncancelled render; unfinished keep the caller's value
+    assert _eq(c[done], full[done])
+    if rc == -4:
+        assert not done.all()
+    else:
+        assert done.all()
     job2 = ctx.submit([(WL.config1_view(), None, None)])
     while not job2.poll():
         pass
@@ -417,3 +478,17 @@ def test_config5_band_at_full_width(ctx, oracle):
         ref = oracle.render(_rect(v), precision=1024, depth=20000, width=3840, height=2160,
                             lines=(l0 + r, 1, 1), cols=(c, 1), threads=1)
         assert int(whole[r, c]) == int(ref[0, 0]), (r, c)
+
+
+def test_config3_whole_batch_persistent_equals_per_tile(ctx):
+    """Config 3's 256 tiles (every precision class, FP64 to 1024 bits, depths
+    1e3 to 5e4, three formulas) in one persistent batch and as 256 launches:
+    identical maps and pixel-iterations."""
+    specs = [(WL.spec_view(sp), None, None) for sp in WL.config3_specs()]
+    print("config 3 persistent", flush=True)
+    a, sa = ctx.render_batch(specs, M.BATCH_PERSISTENT)
+    print("config 3 per tile", flush=True)
+    b, sb = ctx.render_batch(specs, M.BATCH_PER_TILE)
+    for i, (x, y) in enumerate(zip(a, b)):
+        assert _eq(x, y), i
+    assert sa["pixel_iters"] == sb["pixel_iters"]
