@@ -35,13 +35,27 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of k_colour on the 3840x2160 colouring workload by aa:
 # rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 (scaled by a 256 MiB copy's readings)
 COLOUR_TRAFFIC_BYTES = {1: 66.4e6, 2: 166.0e6}
-# HBM bytes per config 2 launch of k_rn_persist<4,true>: rocprofv3 FETCH_SIZE x2 +
-# WRITE_SIZE x1 (both scaled by a 256 MiB copy's readings); the I/O of the
-# kernel does not depend on its arithmetic, so it is measured once
-CONFIG2_TRAFFIC_BYTES = 3.07e6
-# VALU instructions (wave64) per config 2 launch of k_rn_persist<4,true>:
-# rocprofv3 SQ_INSTS_VALU of the current kernel build
-CONFIG2_VALU_INSTS = 7.6552e9
+# rocprofv3 counter figures of k_rn_persist<4,true> on config 2 (HBM bytes per
+# launch from FETCH_SIZE / WRITE_SIZE scaled by a plain copy's readings, VALU
+# instructions per launch from SQ_INSTS_VALU), read at run time from the
+# profile summary they were taken in, which names the command and the build
+CONFIG2_COUNTERS = os.path.join(ROOT, "profiles", "k_rn_persist4_config2_counters.md")
+
+
+def profiled_counters(path):
+    """The `bench-counters: {json}` line of a profile summary, with the file
+    it came from, or None when the file or the line is absent."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("bench-counters:"):
+                    d = json.loads(line.split(":", 1)[1])
+                    d["source"] = os.path.relpath(path, ROOT)
+                    return d
+    except OSError:
+        pass
+    return None
+
 # VALU issue limit: 256 CUs x one wave64 instruction per clock x 2.4 GHz peak clock
 PEAK_VALU_ISSUE = 256 * 2.4e9
 # VALU instructions per wave-iteration of the FP64 kernel's run loop: the
@@ -53,7 +67,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def host_cores():
+def counted_cores():
     """Core count as MDZ reads it (src/main.c:243-273: cpuN lines of /proc/stat)."""
     n = 0
     try:
@@ -63,34 +77,94 @@ def host_cores():
                     n += 1
     except OSError:
         pass
-    n = n or (os.cpu_count() or 1)
-    # a host that shares its cores between GPUs caps a job's threads in OMP_NUM_THREADS
+    return n or (os.cpu_count() or 1)
+
+
+def host_cores():
+    """Threads for the CPU baselines: the counted cores, capped by the job's
+    share of a host that serves several GPUs (OMP_NUM_THREADS)."""
+    n = counted_cores()
     cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
     return min(n, cap) if cap > 0 else n
 
 
-def cpu_baseline(rect, precision, depth, W, H, line_step, threads, path=0):
-    """MDZ's MPFR (or GMP) loop on the host cores, timed on a sample of rows."""
+# how MDZ's loops are built for the CPU baselines (oracle/Makefile)
+REF_FLAGS = {"O0": "gcc -Wall -Wextra -std=gnu99 -ggdb (src/Makefile:11,17, the reference default)",
+             "O2": "gcc -Wall -Wextra -std=gnu99 -O2 -ffp-contract=off"}
+
+
+def cpu_baseline(rect, precision, depth, W, H, line_step, threads, path=0, build="O0",
+                 lines=None, cols=None, sample_name="config 2", repeat=1):
+    """MDZ's loop (MPFR, GMP or long double: src/frac_*.c compiled by
+    oracle/Makefile with the flags of `build`) or the double restatement
+    (path 2, the oracle's -O2 C loop) on the host cores, timed on a sample:
+    every line_step-th row, or the lines x cols crop; repeat times."""
     from oracle import oracle as O
     kind = "reference"
-    try:
-        O.use_reference(True)          # src/frac_*.c compiled as-is (-O0 -ggdb)
-    except Exception:
+    if path == O.PATH_DOUBLE:
         kind = "port"
         O.use_reference(False)
-    nl = (H + line_step - 1) // line_step
-    counts, el = O.render(rect, precision=precision, depth=depth, width=W, height=H,
-                          lines=(0, nl, line_step), threads=threads, timed=True, path=path)
+    else:
+        try:
+            O.use_reference(True, build)
+        except Exception:
+            kind = "port"
+            O.use_reference(False)
+    if lines is None:
+        nl = (H + line_step - 1) // line_step
+        lines = (0, nl, line_step)
+    el, pi = 0.0, 0.0
+    for _ in range(repeat):
+        counts, e = O.render(rect, precision=precision, depth=depth, width=W, height=H,
+                             lines=lines, cols=cols, threads=threads, timed=True, path=path)
+        el += e
+        pi += float(((counts > 0) * counts + (counts == 0) * depth).sum())
     O.use_reference(False)
-    pi = float(((counts > 0) * counts + (counts == 0) * depth).sum())
+    where = (f"rows {lines[0]},{lines[0] + lines[2]},.. ({lines[1]} rows)" if cols is None
+             else f"{lines[1]}x{cols[1]} crop at row {lines[0]} col {cols[0]}")
     return dict(value=pi / el / 1e9, unit="G pixel-iter/s", cores=threads, kind=kind,
-                sample=f"config 2 rows 0,{line_step},..({nl} rows x {W} px, {pi:.3e} "
-                       f"pixel-iterations) in {el:.2f} s, {threads} threads claiming rows")
+                build=(REF_FLAGS[build] if kind == "reference" else
+                       "oracle restatement, gcc -O2 -ffp-contract=off"),
+                counted_cores=counted_cores(),
+                sample=f"{sample_name} {where} x {W if cols is None else cols[1]} px, "
+                       f"{pi:.3e} pixel-iterations{' x' + str(repeat) if repeat > 1 else ''} "
+                       f"in {el:.2f} s, {threads} threads claiming rows")
 
 
-def config3_batched(ctx, M, WL, np, ntiles=256):
+def config3_classes(ctx, M, WL, np, specs):
+    """Config 3 by arithmetic class: each class's tiles alone, as one
+    persistent launch and as one launch per tile.  The persistent batch of
+    all tiles cannot take less than the sum of the classes' persistent times
+    less what running the classes concurrently overlaps, and the per-tile
+    form takes the sum of its classes' per-tile times; their ratio bounds
+    what batching gains on this tile distribution."""
+    by = {}
+    for sp in specs:
+        by.setdefault(sp["precision"], []).append(sp)
+    out = {}
+    for p in sorted(by):
+        tiles = [(WL.spec_view(sp), None, None) for sp in by[p]]
+        _, sp_ = ctx.render_batch(tiles, M.BATCH_PERSISTENT)
+        _, st_ = ctx.render_batch(tiles, M.BATCH_PER_TILE)
+        npix = sum(sp["side"] ** 2 for sp in by[p])
+        out[str(p)] = dict(tiles=len(by[p]), pixels=npix, pixel_iters=sp_["pixel_iters"],
+                           persistent_ms=sp_["kernel_ms"], per_tile_ms=st_["kernel_ms"],
+                           persistent_gpix_per_s=sp_["pixel_iters"] / sp_["kernel_ms"] / 1e6,
+                           per_tile_gpix_per_s=st_["pixel_iters"] / st_["kernel_ms"] / 1e6,
+                           per_tile_over_persistent=st_["kernel_ms"] / sp_["kernel_ms"])
+        log(f"config 3 class p={p}: {len(by[p])} tiles, persistent {sp_['kernel_ms']:.0f} ms, "
+            f"per-tile {st_['kernel_ms']:.0f} ms")
+    sp_sum = sum(v["persistent_ms"] for v in out.values())
+    st_sum = sum(v["per_tile_ms"] for v in out.values())
+    return dict(classes=out, sum_class_persistent_ms=sp_sum, sum_class_per_tile_ms=st_sum,
+                bound_per_tile_over_persistent=st_sum / sp_sum)
+
+
+def config3_batched(ctx, M, WL, np, ntiles=256, classes=False, graphs_off=False):
     """Config 3: the seeded tiles in one persistent launch per arithmetic class
-    (concurrent classes) against one plain launch per tile; counts must match."""
+    (concurrent classes) against one plain launch per tile; counts must match.
+    classes: also each class alone (config3_classes); graphs_off: also the
+    persistent batch with its launches made one by one."""
     specs = WL.config3_specs(ntiles)
     t0 = time.perf_counter()
     tiles = [(WL.spec_view(sp), None, None) for sp in specs]
@@ -105,6 +179,16 @@ def config3_batched(ctx, M, WL, np, ntiles=256):
         by[sp["precision"]] = by.get(sp["precision"], 0) + 1
     gp = st_p["pixel_iters"] / (st_p["wall_ms"] * 1e-3) / 1e9
     gt = st_t["pixel_iters"] / (st_t["wall_ms"] * 1e-3) / 1e9
+    extra = {}
+    if graphs_off:
+        ctx.option_set(M.OPT_GRAPHS, 0)
+        _, st_e = ctx.render_batch(tiles, M.BATCH_PERSISTENT)
+        ctx.option_set(M.OPT_GRAPHS, 1)
+        extra["persistent_without_graph"] = dict(kernel_ms=st_e["kernel_ms"],
+                                                 wall_ms=st_e["wall_ms"])
+        log(f"config 3 persistent, launches one by one: {st_e['wall_ms']:.0f} ms wall")
+    if classes:
+        extra["by_class"] = config3_classes(ctx, M, WL, np, specs)
     return dict(
         metric=METRIC, value=gp, steps=1, warmup=0, ms_per_step=st_p["wall_ms"],
         dtype="u32 limbs RN p in {64..1024} + f64",
@@ -115,7 +199,7 @@ def config3_batched(ctx, M, WL, np, ntiles=256):
         batched=dict(persistent_gpix_per_s=gp, per_tile_gpix_per_s=gt, speedup=gp / gt,
                      persistent_kernel_ms=st_p["kernel_ms"], per_tile_kernel_ms=st_t["kernel_ms"],
                      launches_persistent=st_p["launches"], launches_per_tile=st_t["launches"],
-                     identical_counts=bool(same)))
+                     identical_counts=bool(same), **extra))
 
 
 def gmp_path(ctx, M, WL, a, cpu=True):
@@ -267,6 +351,15 @@ def config4_serving(ctx, M, WL, np, fracs=(0.25, 0.5, 0.75, 0.9), n=300):
     log(f"serve batched, image requests: saturation {img['tiles_per_s']:.0f} tiles/s "
         f"{img['gpix_per_s']:.1f} G pixel-iter/s")
     out["batched_image_saturation"] = img
+    # the batched form with each slice's launches made one by one (no graphs)
+    ctx.option_set(M.OPT_GRAPHS, 0)
+    unl0 = unloaded_latency(ctx, M, np, M.SERVE_BATCHED, tiles)
+    sat0 = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, float("inf"))
+    half0 = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, 0.5 * sat0["tiles_per_s"])
+    ctx.option_set(M.OPT_GRAPHS, 1)
+    out["batched_without_graphs"] = dict(unloaded_ms=unl0, saturation=sat0, half_load=half0)
+    log(f"serve batched without graphs: unloaded {unl0:.2f} ms, saturation "
+        f"{sat0['gpix_per_s']:.1f} G pixel-iter/s, half load p99 {half0['p99_ms']:.2f} ms")
     return out
 
 
@@ -432,23 +525,27 @@ def main():
         kavg = sum(kms) / len(kms)
         ops = WL.ops_per_pixel_iter(M.PATH_MPFR, 128)
         achieved = 2.0 * ops * pi_step / (kavg * 1e-3) / 1e12
+        pc = profiled_counters(CONFIG2_COUNTERS)
+        roof = dict(bound="valu", achieved=achieved, peak=PEAK_TOPS, unit="TOP/s",
+                    frac=achieved / PEAK_TOPS, traffic=pc["traffic_bytes"] if pc else None,
+                    kernel_ms=kavg,
+                    ops="2n^2+n = 36 algorithmic 32x32-bit MACs (x2 ops) per pixel-iteration "
+                        "(n = 4) x 1.536e9 per launch")
+        if pc:
+            vi = pc["valu_insts_per_launch"]
+            roof["counters"] = dict(source=pc["source"], build=pc.get("build"),
+                                    note="rocprofv3 --pmc of the build named, read from the "
+                                         "profile summary at run time")
+            roof["valu_issue"] = dict(insts_per_launch=vi, achieved=vi / (kavg * 1e-3),
+                                      peak=PEAK_VALU_ISSUE, unit="wave64 VALU instructions/s",
+                                      frac=vi / (kavg * 1e-3) / PEAK_VALU_ISSUE)
         res.update(value=sum(pis) / el / 1e9, steps=a.steps, warmup=a.warmup,
                    ms_per_step=1e3 * el / a.steps, dtype="u32 limbs, RN p=128",
                    config=dict(workload="config2 Mandelbrot MPFR-path p=128 640x480 "
                                "centre (-0.743643887037158704752191506114774, "
                                "0.131825904205311970493132056385139) width 1e-25 depth 5000",
                                pixel_iters_per_step=pi_step, kernel="k_rn_persist<4,true>"),
-                   kernel_gpix_per_s=pi_step / (kavg * 1e-3) / 1e9,
-                   roofline=dict(bound="valu", achieved=achieved, peak=PEAK_TOPS, unit="TOP/s",
-                                 frac=achieved / PEAK_TOPS, traffic=CONFIG2_TRAFFIC_BYTES,
-                                 kernel_ms=kavg,
-                                 ops="2n^2+n = 36 algorithmic 32x32-bit MACs (x2 ops) per pixel-iteration (n = 4) "
-                                     "x 1.536e9 per launch",
-                                 valu_issue=dict(
-                                     insts_per_launch=CONFIG2_VALU_INSTS,
-                                     achieved=CONFIG2_VALU_INSTS / (kavg * 1e-3),
-                                     peak=PEAK_VALU_ISSUE, unit="wave64 VALU instructions/s",
-                                     frac=CONFIG2_VALU_INSTS / (kavg * 1e-3) / PEAK_VALU_ISSUE)))
+                   kernel_gpix_per_s=pi_step / (kavg * 1e-3) / 1e9, roofline=roof)
 
         if a.headline_only:
             ctx.close()
@@ -471,6 +568,19 @@ def main():
                            kernel_gpix_per_s=p1 / (k1a * 1e-3) / 1e9,
                            frac_fp64_peak=7 * p1 / (k1a * 1e-3) / 1e12 / PEAK_TOPS)
         log(f"fp64: {res['fp64']['kernel_gpix_per_s']:.1f} G pixel-iter/s (kernel)")
+        # the long double path (RN at 64 bits) on the same frame
+        vld = WL.config1_view(path=M.PATH_LD)
+        for _ in range(3):
+            ctx.render(vld)
+        kl = []
+        for _ in range(20):
+            _, st = ctx.render(vld)
+            kl.append(st["kernel_ms"])
+        kla = sum(kl) / len(kl)
+        res["fp64"]["ld_path"] = dict(workload="config1 240x180 depth 1000, long double path "
+                                      "(round-to-nearest at 64 bits)",
+                                      pixel_iters=st["pixel_iters"], kernel_ms=kla,
+                                      kernel_gpix_per_s=st["pixel_iters"] / (kla * 1e-3) / 1e9)
         # the FP64 kernel on a frame that fills the device: the same view at 3840x2160
         v4 = WL.config1_view(W=3840, H=2160)
         ctx.render(v4)
@@ -493,28 +603,91 @@ def main():
         res["rn1024"] = rn1024_band(ctx, WL, 192)
         log(f"rn1024: {res['rn1024']['kernel_gpix_per_s']:.3f} G pixel-iter/s (kernel)")
 
-        # p50/p99 tile latency: continuous batching, 64x64 config 2 crops
+        # p50/p99 tile latency: continuous batching, 64x64 config 2 crops,
+        # against one launch per request (the north star's serving claim:
+        # >= 3x the per-request form's throughput at p99 <= 2x unloaded)
         tiles, _ = crop_tiles(WL, 300)
         unl = unloaded_latency(ctx, M, np, M.SERVE_BATCHED, tiles)
         sat = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, float("inf"))
-        half = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, 0.5 * sat["tiles_per_s"])
+        unl_pr = unloaded_latency(ctx, M, np, M.SERVE_PER_REQUEST, tiles)
+        sat_pr = serve_run(ctx, M, np, M.SERVE_PER_REQUEST, tiles, float("inf"))
+        loads = []
+        for f in (0.5, 0.6, 0.7, 0.8, 0.9):
+            r = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, f * sat["tiles_per_s"])
+            loads.append(dict(load=f, rate=r["rate"], p50_ms=r["p50_ms"], p99_ms=r["p99_ms"],
+                              tiles_per_s=r["tiles_per_s"], gpix_per_s=r["gpix_per_s"],
+                              p99_over_unloaded=r["p99_ms"] / unl))
+        ok = [x for x in loads if x["p99_over_unloaded"] <= 2.0]
+        best = max(ok, key=lambda x: x["gpix_per_s"]) if ok else None
+        half = loads[0]
         res["latency"] = dict(workload="Poisson requests of 64x64 crops of config 2 (p=128, "
                               "depth 5000), continuous batching", unloaded_p50_ms=unl,
                               saturation_tiles_per_s=sat["tiles_per_s"],
                               saturation_gpix_per_s=sat["gpix_per_s"],
                               at_half_load=dict(rate=half["rate"], p50_ms=half["p50_ms"],
                                                 p99_ms=half["p99_ms"],
-                                                tiles_per_s=half["tiles_per_s"]))
+                                                tiles_per_s=half["tiles_per_s"]),
+                              sweep=loads,
+                              per_request=dict(unloaded_p50_ms=unl_pr,
+                                               saturation_tiles_per_s=sat_pr["tiles_per_s"],
+                                               saturation_gpix_per_s=sat_pr["gpix_per_s"]),
+                              north_star=dict(
+                                  load=best["load"] if best else None,
+                                  p99_over_unloaded=best["p99_over_unloaded"] if best else None,
+                                  batched_gpix_per_s=best["gpix_per_s"] if best else None,
+                                  per_request_saturation_gpix_per_s=sat_pr["gpix_per_s"],
+                                  ratio=best["gpix_per_s"] / sat_pr["gpix_per_s"] if best else None,
+                                  target="ratio >= 3 at p99 <= 2x unloaded latency"))
         log(f"latency: unloaded {unl:.2f} ms; half load p50 {half['p50_ms']:.2f} "
-            f"p99 {half['p99_ms']:.2f} ms")
+            f"p99 {half['p99_ms']:.2f} ms; per-request saturation {sat_pr['gpix_per_s']:.1f} G; "
+            f"north star ratio {res['latency']['north_star']['ratio']}")
+
+        # config 3: 256 mixed tiles, persistent batch against one launch per tile
+        c3 = config3_batched(ctx, M, WL, np)
+        res["config3"] = dict(workload=c3["config"]["workload"], **c3["batched"])
 
         if not a.no_cpu_baseline:
-            log("cpu baseline (MDZ MPFR loop on host cores) ...")
+            log("cpu baseline (MDZ's loops on host cores) ...")
+            th = host_cores()
             rect = [s.decode() for s in v._keep[:4]]
-            res["cpu_baseline"] = cpu_baseline(rect, 128, 5000, 640, 480, 4, host_cores())
+            legs = {}
+            # 128 bits: config 2, every 4th row
+            for b in ("O0", "O2"):
+                legs[f"rn128_{b}"] = cpu_baseline(rect, 128, 5000, 640, 480, 4, th, build=b)
+                log(f"cpu 128-bit {b}: {legs[f'rn128_{b}']['value']:.4f} G pixel-iter/s")
+            # 1024 bits: a seeded 64x64 crop of config 5
+            import random
+            v5 = WL.config5_view()
+            rng = random.Random(5)
+            l0, c0 = rng.randrange(v5.real_height - 64), rng.randrange(v5.real_width - 64)
+            r5 = [s_.decode() for s_ in v5._keep[:4]]
+            for b in ("O0", "O2"):
+                legs[f"rn1024_{b}"] = cpu_baseline(r5, 1024, 20000, v5.real_width,
+                                                   v5.real_height, 1, th, build=b,
+                                                   lines=(l0, 64, 1), cols=(c0, 64),
+                                                   sample_name="config 5")
+                log(f"cpu 1024-bit {b}: {legs[f'rn1024_{b}']['value']:.4f} G pixel-iter/s")
+            # long double (MDZ's "double" path) and the IEEE double loop: config 1
+            r1 = [s_.decode() for s_ in v1._keep[:4]]
+            for b in ("O0", "O2"):
+                legs[f"ld_{b}"] = cpu_baseline(r1, 64, 1000, 240, 180, 1, th, path=M.PATH_LD,
+                                               build=b, sample_name="config 1", repeat=30)
+                log(f"cpu long double {b}: {legs[f'ld_{b}']['value']:.3f} G pixel-iter/s")
+            legs["double_O2"] = cpu_baseline(r1, 64, 1000, 240, 180, 1, th,
+                                             path=M.PATH_DOUBLE, sample_name="config 1",
+                                             repeat=30)
+            log(f"cpu double: {legs['double_O2']['value']:.3f} G pixel-iter/s")
+            gpu = dict(rn128=res["value"], rn1024=res["rn1024"]["kernel_gpix_per_s"],
+                       ld=res["fp64"]["ld_path"]["kernel_gpix_per_s"],
+                       double=res["fp64"]["kernel_gpix_per_s"])
+            ratios = {k: gpu[k.rsplit("_", 1)[0]] / x["value"] for k, x in legs.items()}
+            res["cpu_baseline"] = dict(legs["rn128_O0"])
+            res["cpu_baseline"]["legs"] = legs
+            res["cpu_baseline"]["gpu_over_cpu"] = ratios
+            res["cpu_baseline"]["gpu_figures"] = gpu
             log(f"cpu baseline: {res['cpu_baseline']['value']:.4f} G pixel-iter/s")
     elif a.config == "3":
-        res.update(config3_batched(ctx, M, WL, np))
+        res.update(config3_batched(ctx, M, WL, np, classes=True, graphs_off=True))
     elif a.config == "gmp":
         res.update(gmp_path(ctx, M, WL, a, cpu=not a.no_cpu_baseline))
     elif a.config == "colour":

@@ -256,6 +256,45 @@ int mdzg_tune_set(mdzg_ctx* ctx, int kernel_id, int blocks_per_cu, int chunk);
 int mdzg_tune_get(mdzg_ctx* ctx, int kernel_id, int* blocks_per_cu, int* chunk,
                   int* max_blocks_per_cu);
 
+/* ---- options (no counterpart in the reference) --------------------------
+   MDZG_OPT_GRAPHS        1 (default): a persistent batch (its sentinel fill,
+                          its concurrent class launches and an image job's
+                          colouring pass) and each serving slice (counter
+                          reset and class launches) run as one replay of a
+                          cached HIP graph; 0: launched one by one on streams.
+   MDZG_OPT_BAND_PIXELS   the line adapter's band size in pixels (a band is
+                          whole rows, at least this many pixels); 1..2^30.
+   MDZG_OPT_BAND_INFLIGHT bands the line adapter keeps in flight; 1..16.
+   Options set before mdzg_frame_open or mdzg_server_create apply to it. */
+enum { MDZG_OPT_GRAPHS = 1, MDZG_OPT_BAND_PIXELS = 2, MDZG_OPT_BAND_INFLIGHT = 3 };
+int mdzg_option_set(mdzg_ctx* ctx, int option, long long value);
+int mdzg_option_get(mdzg_ctx* ctx, int option, long long* value);
+
+/* ---- autotuning (no counterpart in the reference) -----------------------
+   mdzg_autotune renders `probes` (1 to 8 tiles of the kernel class to tune,
+   e.g. one whose pixels escape at many iterations and one whose pixels all
+   run to depth) at every candidate pair of blocks per CU (1 .. the kernel's
+   occupancy limit) and chunk (iterations between refills: 1, 2, 4, .. 32 on
+   the RN and GMP paths, 64 .. 1024 on FP64), `reps` times each, timing the
+   kernel with HIP events.  A pair's cost is the mean over the probes of its
+   time over the time of the pair in place; the cheapest pair is kept in the
+   context (mdzg_tune_set) when its cost is below 0.98, beyond the probes'
+   run-to-run spread.  best_ms and default_ms sum the probes' times.
+   mdzg_tune_save writes the context's table (every kernel's pair and the
+   options above) as text; mdzg_tune_load reads one.  mdzg_create loads the
+   table named by the environment variable MDZG_TUNE_TABLE, else the file
+   tune_gfx950.txt beside the library, when it exists. */
+typedef struct mdzg_tune_result {
+    int kernel_id;
+    int blocks_per_cu, chunk;         /* the pair kept                         */
+    double best_ms, default_ms;       /* its probes' kernel time, the previous pair's */
+    int candidates;                   /* pairs measured                        */
+} mdzg_tune_result;
+int mdzg_autotune(mdzg_ctx* ctx, const mdzg_tile* probes, int nprobes, int reps,
+                  mdzg_tune_result* out);
+int mdzg_tune_save(mdzg_ctx* ctx, const char* path);
+int mdzg_tune_load(mdzg_ctx* ctx, const char* path);
+
 /* ---- asynchronous form (submit / poll / wait / cancel) ---------------- */
 typedef struct mdzg_job mdzg_job;
 int  mdzg_submit(mdzg_ctx* ctx, const mdzg_tile* tiles, int ntiles, mdzg_job** out);

@@ -81,6 +81,18 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("kernel_id", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
+                ("chunk", ctypes.c_int), ("best_ms", ctypes.c_double),
+                ("default_ms", ctypes.c_double), ("candidates", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+OPT_GRAPHS, OPT_BAND_PIXELS, OPT_BAND_INFLIGHT = 1, 2, 3
+
+
 class ServerCfg(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
                 ("occupancy_budget", ctypes.c_double), ("slice_iters", ctypes.c_int),
@@ -166,6 +178,11 @@ def lib():
         L.mdzg_server_stats.argtypes = [vp, P(ServeStats)]
         L.mdzg_server_destroy.argtypes = [vp]
         L.mdzg_server_destroy.restype = None
+        L.mdzg_option_set.argtypes = [vp, ctypes.c_int, ctypes.c_longlong]
+        L.mdzg_option_get.argtypes = [vp, ctypes.c_int, P(ctypes.c_longlong)]
+        L.mdzg_autotune.argtypes = [vp, P(Tile), ctypes.c_int, ctypes.c_int, P(TuneResult)]
+        L.mdzg_tune_save.argtypes = [vp, ctypes.c_char_p]
+        L.mdzg_tune_load.argtypes = [vp, ctypes.c_char_p]
         L.mdzg_kernel_id.argtypes = [ctypes.c_int, ctypes.c_long]
         L.mdzg_tune_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.mdzg_tune_get.argtypes = [vp, ctypes.c_int] + [P(ctypes.c_int)] * 3
@@ -396,6 +413,34 @@ class Context:
 
     def tune_set(self, kernel_id, blocks_per_cu=0, chunk=0):
         _check(lib().mdzg_tune_set(self._c, kernel_id, blocks_per_cu, chunk), "tune_set")
+
+    def option_set(self, option, value):
+        _check(lib().mdzg_option_set(self._c, option, int(value)), "option_set")
+
+    def option_get(self, option):
+        v = ctypes.c_longlong()
+        _check(lib().mdzg_option_get(self._c, option, ctypes.byref(v)), "option_get")
+        return v.value
+
+    def autotune(self, views, reps=2):
+        """mdzg_autotune on whole-view probes (a view or a list of views of one
+        kernel class): the kept pair and times."""
+        views = views if isinstance(views, (list, tuple)) else [views]
+        arr = (Tile * len(views))()
+        keep = []
+        for i, v in enumerate(views):
+            t, out = self._tile(v, None, None)
+            arr[i] = t
+            keep.append(out)
+        r = TuneResult()
+        _check(lib().mdzg_autotune(self._c, arr, len(views), reps, ctypes.byref(r)), "autotune")
+        return r.as_dict()
+
+    def tune_save(self, path):
+        _check(lib().mdzg_tune_save(self._c, _b(path)), "tune_save")
+
+    def tune_load(self, path):
+        _check(lib().mdzg_tune_load(self._c, _b(path)), "tune_load")
 
     def submit(self, specs, fill=0):
         """Asynchronous batch: returns a Job.  The counts buffers start at

@@ -42,7 +42,7 @@ C_FLAGS = [
 # the sonames (libmpfr.so.6, libgmp.so.10) fix the ABI.
 SYS_LIB = os.environ.get("MDZG_SYS_LIBDIR", "/usr/lib/x86_64-linux-gnu")
 LINK_LIBS = [f"-L{SYS_LIB}", "-l:libmpfr.so.6", "-l:libgmp.so.10", "-l:libz.so.1",
-             "-lpthread", "-lm"]
+             "-lpthread", "-lm", "-ldl"]
 
 
 def _run(cmd):
@@ -65,6 +65,8 @@ def jobs():
                    "-o", os.path.join(BUILD, "k_colour.o")], "k_colour.o"))
     steps.append(([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "noalloc.hip"),
                    "-o", os.path.join(BUILD, "noalloc.o")], "noalloc.o"))
+    steps.append(([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "graph.hip"),
+                   "-o", os.path.join(BUILD, "graph.o")], "graph.o"))
     for n in sorted(NWORDS, reverse=True):   # largest (slowest) first
         steps.append(([hipcc, *HIP_FLAGS, f"-DMDZG_N={n}", "-c", os.path.join(CSRC, "k_rn.hip"),
                        "-o", os.path.join(BUILD, f"k_rn{n}.o")], f"k_rn{n}.o"))

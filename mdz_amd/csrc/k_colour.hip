@@ -31,8 +31,10 @@
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "colour_math.h"
+#include "mdzg_dev.h"
 
 using mdzg::colour::colour_apply;
 using mdzg::colour::colour_get;
@@ -191,14 +193,15 @@ __global__ __launch_bounds__(256) void k_colour(colour_args a)
 
 typedef void (*kfn)(colour_args);
 
-extern "C" int mdzg_dev_launch_colour(const uint32_t* counts, int real_w, int user_w, int user_h,
-                                      int aa, const uint32_t* palette, int n, int offset,
-                                      double sc, int ip, uint32_t depth, uint32_t* rgb,
-                                      unsigned long long* iters, int max_blocks, hipStream_t st)
+/* the colouring pass as a launch node (mdzg_dev.h) */
+extern "C" int mdzg_dev_node_colour(mdzg_gnode* node, const uint32_t* counts, int real_w,
+                                    int user_w, int user_h, int aa, const uint32_t* palette,
+                                    int n, int offset, double sc, int ip, uint32_t depth,
+                                    uint32_t* rgb, unsigned long long* iters, int max_blocks)
 {
-    if (!counts || !palette || !rgb || user_w < 1 || user_h < 1 || aa < 1 || n < 2 || n > 256 ||
-        real_w < user_w * aa || (uint64_t)user_w * (uint64_t)user_h > 0x7fffffffull ||
-        max_blocks < 1)
+    if (!node || !counts || !palette || !rgb || user_w < 1 || user_h < 1 || aa < 1 || n < 2 ||
+        n > 256 || real_w < user_w * aa ||
+        (uint64_t)user_w * (uint64_t)user_h > 0x7fffffffull || max_blocks < 1)
         return -1;
     colour_args a;
     a.counts = counts;
@@ -239,6 +242,28 @@ extern "C" int mdzg_dev_launch_colour(const uint32_t* counts, int real_w, int us
         { { k_colour<2, false, false>, k_colour<2, false, true> },
           { k_colour<2, true, false>, k_colour<2, true, true> } },
     };
-    hipLaunchKernelGGL(table[form][ip ? 1 : 0][iters ? 1 : 0], dim3(grid), dim3(256), 0, st, a);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    static_assert(sizeof(colour_args) <= MDZG_GNODE_BUF, "colour_args fits a launch node");
+    memset(node, 0, sizeof *node);
+    node->func = (const void*)table[form][ip ? 1 : 0][iters ? 1 : 0];
+    node->grid = grid;
+    node->block = 256;
+    node->nargs = 1;
+    node->off[0] = 0;
+    node->used = (int)sizeof a;
+    memcpy(node->buf, &a, sizeof a);
+    return 0;
+}
+
+extern "C" int mdzg_dev_node_launch(const mdzg_gnode* n, hipStream_t st);
+
+extern "C" int mdzg_dev_launch_colour(const uint32_t* counts, int real_w, int user_w, int user_h,
+                                      int aa, const uint32_t* palette, int n, int offset,
+                                      double sc, int ip, uint32_t depth, uint32_t* rgb,
+                                      unsigned long long* iters, int max_blocks, hipStream_t st)
+{
+    mdzg_gnode node;
+    if (mdzg_dev_node_colour(&node, counts, real_w, user_w, user_h, aa, palette, n, offset, sc,
+                             ip, depth, rgb, iters, max_blocks) != 0)
+        return -1;
+    return mdzg_dev_node_launch(&node, st) == 0 ? 0 : -1;
 }

@@ -342,14 +342,30 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
            1000). */
         const uint64_t am = __ballot(active);
         const int lead = __ffsll((long long)am) - 1;
-        const int f0 = __shfl(fractal, lead);
+        /* a scalar, so that the formula switch below is a uniform branch and
+           the lane masks stay in scalar registers across it */
+        const int f0 = __builtin_amdgcn_readfirstlane(__shfl(fractal, lead));
         if (__all(!active || fractal == f0)) {
-            const int rem = __ockl_wfred_min_i32(active ? depth - wz : 0x7fffffff);
-            uint64_t live = am, escm = 0;
-            if (rem >= chunk)
+            /* the wave minimum, read back as a scalar so that the compiler
+               keeps the run loops' bounds, and the lane masks they update,
+               uniform (scalar registers) */
+            const int rem = __builtin_amdgcn_readfirstlane(
+                __ockl_wfred_min_i32(active ? depth - wz : 0x7fffffff));
+            /* add_lanes' asm reads live: a scalar copy of the ballot (later
+               iterations update it with scalar instructions) */
+            uint64_t live = scalar_copy(am), escm = 0;
+            if (rem >= chunk) {
                 fp64_run_formula<false>(f0, s, wz, live, escm, depth, chunk);
-            else
-                fp64_run_formula<true>(f0, s, wz, live, escm, depth, chunk);
+            } else {
+                /* no live lane passes its depth in the first rem iterations
+                   (rem >= 1: an active lane has wz < depth): those run without
+                   the depth test; lanes that reached their depth there leave
+                   the mask (count 0 unless they escaped, which already took
+                   them out), and the rest of the run has the test */
+                fp64_run_formula<false>(f0, s, wz, live, escm, depth, rem);
+                live &= ~__builtin_amdgcn_ballot_w64(wz >= depth);
+                fp64_run_formula<true>(f0, s, wz, live, escm, depth, chunk - rem);
+            }
             const bool esc = (escm >> __lane_id()) & 1u;
             if (active && (esc || wz >= depth)) {
                 put_count(out, esc ? (uint32_t)wz : 0u);

@@ -7,6 +7,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "mdzg_dev.h"
 
@@ -46,9 +47,84 @@ static void lookup(int kid, int form, const void** f)
     *f = form == 2 ? sl : (form == 1 ? p : t);
 }
 
+/* counts to the UINT_MAX sentinel, queue counters to zero: a kernel, so
+   that a graph holds no memset node */
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = v;
+}
+
+static void node_init(mdzg_gnode* n, const void* f, unsigned grid, unsigned block)
+{
+    memset(n, 0, sizeof *n);
+    n->func = f;
+    n->grid = grid;
+    n->block = block;
+}
+
+template <class T>
+static void node_arg(mdzg_gnode* n, const T& v)
+{
+    static_assert(sizeof(T) <= MDZG_GNODE_BUF, "argument too large");
+    const int o = (n->used + 7) & ~7;
+    memcpy((char*)n->buf + o, &v, sizeof v);
+    n->off[n->nargs++] = o;
+    n->used = o + (int)sizeof v;
+}
+
 extern "C" {
 
 int mdzg_dev_nkernels(void) { return MDZG_NKERNELS; }
+
+int mdzg_dev_node_fill(mdzg_gnode* n, uint32_t* p, uint64_t count, uint32_t v)
+{
+    if (!n || !p || count == 0)
+        return -1;
+    uint64_t g = (count + 255) / 256;
+    if (g > 2048)
+        g = 2048;
+    node_init(n, (const void*)k_fill_u32, (unsigned)g, 256);
+    node_arg(n, p);
+    node_arg(n, count);
+    node_arg(n, v);
+    return 0;
+}
+
+int mdzg_dev_node_persist(mdzg_gnode* n, int kid, const mdzg_dtile* d_tiles, int ntiles,
+                          uint64_t total, unsigned long long* d_counter, const int* d_cancel,
+                          int grid, int chunk)
+{
+    if (!n || kid < 0 || kid >= mdzg_dev_nkernels() || ntiles <= 0 || grid <= 0 || chunk <= 0)
+        return -1;
+    const void* f;
+    lookup(kid, 1, &f);
+    node_init(n, f, (unsigned)grid, 256);
+    node_arg(n, d_tiles);
+    node_arg(n, ntiles);
+    node_arg(n, total);
+    node_arg(n, d_counter);
+    node_arg(n, d_cancel);
+    node_arg(n, chunk);
+    return 0;
+}
+
+int mdzg_dev_node_slice(mdzg_gnode* n, int kid, const mdzg_dtile* d_tiles, int ntiles,
+                        uint64_t total, unsigned long long* d_counter, int grid, int budget)
+{
+    if (!n || kid < 0 || kid >= mdzg_dev_nkernels() || ntiles <= 0 || grid <= 0 || budget <= 0)
+        return -1;
+    const void* f;
+    lookup(kid, 2, &f);
+    node_init(n, f, (unsigned)grid, 256);
+    node_arg(n, d_tiles);
+    node_arg(n, ntiles);
+    node_arg(n, total);
+    node_arg(n, d_counter);
+    node_arg(n, budget);
+    return 0;
+}
 
 int mdzg_dev_occupancy(int kid, int persist)
 {

@@ -55,4 +55,19 @@ typedef struct mdzg_dtile {
     unsigned int* remaining;    /* unfinished pixels of this request                 */
 } mdzg_dtile;
 
+/* One kernel launch as data: the kernel, its grid, and its arguments packed
+   in buf at off[i] (each aligned to 8 bytes).  The runtime builds launches
+   as nodes and either launches them at once or replays them as the nodes of
+   a cached HIP graph (graph.hip). */
+#define MDZG_GNODE_ARGS 8
+#define MDZG_GNODE_BUF 160
+typedef struct mdzg_gnode {
+    const void* func;
+    unsigned grid, block, shmem;
+    int nargs;
+    int used;                   /* bytes of buf in use */
+    int off[MDZG_GNODE_ARGS];
+    uint64_t buf[MDZG_GNODE_BUF / 8];
+} mdzg_gnode;
+
 #endif

@@ -25,9 +25,9 @@
  * word-aligned rounding path, kclr > 0 the masked one.
  *
  * gfx950 mapping: a 32x32+64 multiply-accumulate is one v_mad_u64_u32 whose
- * carry-out feeds one v_addc_co_u32 into the third accumulator word (the
- * device form of mac96 below; the host form computes the same function in
- * portable C).  Word carry chains use v_add_co/v_addc_co via __builtin_addc.
+ * carry-out feeds one v_addc_co_u32 into the third accumulator word, issued
+ * in groups that keep the carry's read two instructions after its write
+ * (macg below; the host form computes the same function in portable C).  Word carry chains use v_add_co/v_addc_co via __builtin_addc.
  * Rare cases (carry through a rounding increment, cancellation of a whole
  * word, an alignment shift of whole words) run under wave-wide votes, so a wave
  * only pays for them when one of its lanes needs them.  Built for the host
@@ -67,11 +67,29 @@ MP_HD bool wave_any(bool c)
 #endif
 }
 
-/* lane mask of a per-lane predicate (device) / the predicate itself (host) */
+/* A lane mask copied by the scalar unit.  A compare writes its lane mask
+   with a VALU instruction, and gfx950 needs two wait states before a VALU
+   instruction reads an SGPR that a VALU instruction wrote; the compiler
+   inserts them for its own code but not before an asm block.  The asm
+   blocks below that read a mask read this copy, which an SALU instruction
+   wrote: that read needs no wait. */
+MP_HD uint64_t scalar_copy(uint64_t m)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t r;
+    asm("s_mov_b64 %0, %1" : "=s"(r) : "s"(m));
+    return r;
+#else
+    return m;
+#endif
+}
+
+/* lane mask of a per-lane predicate (device, a scalar copy for the asm
+   readers) / the predicate itself (host) */
 MP_HD uint64_t lane_mask(bool c)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_ballot_w64(c);
+    return scalar_copy(__builtin_amdgcn_ballot_w64(c));
 #else
     return c ? 1u : 0u;
 #endif
@@ -188,7 +206,7 @@ template <int N>
 MP_HD void add_one_if(uint32_t* R, int32_t& e, bool inc)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    const uint64_t m = __builtin_amdgcn_ballot_w64(inc);
+    const uint64_t m = scalar_copy(__builtin_amdgcn_ballot_w64(inc));
     uint64_t co;
     uint32_t r0;
     asm("v_addc_co_u32 %0, %1, 0, %2, %3" : "=v"(r0), "=s"(co) : "v"(R[0]), "s"(m));
@@ -290,87 +308,255 @@ MP_HD void round_p(uint32_t* R, uint32_t G, uint32_t S, int32_t& e, int kclr)
 /* Outputs the top N+1 words of the 2N-word product (H[0] = word N-1,   */
 /* the guard word; H[N] = word 2N-1) and the OR of words 0..N-2.        */
 /* Each column sum is kept in a 96-bit accumulator (c2 : acc).          */
-/*                                                                      */
-/* In the asm blocks below v_addc_co_u32 reads, as its carry-in, the    */
-/* SGPR pair that the v_mad_u64_u32 right before it wrote.  The compiler */
-/* keeps two wait states between a VALU's SGPR write and a VALU read of  */
-/* it; a wave64 VALU instruction takes four cycles on a 16-lane SIMD,    */
-/* so they have elapsed when the next instruction of the wave issues.    */
-/* The products are checked bit for bit against MPFR on the device;      */
-/* an explicit s_nop 1 before each carry read measured 1 % slower at 4   */
-/* words and 20 % at 32 words, with identical results.                   */
 /* ------------------------------------------------------------------ */
+
+/* Multiply-accumulate groups.  A product a*b joins the column sum with one
+   v_mad_u64_u32 (acc += a*b, its carry-out to an SGPR pair) and one
+   v_addc_co_u32 that adds that carry into the third word c2.  gfx950 needs
+   two wait states between a VALU write of an SGPR and a VALU read of it,
+   and the compiler inserts none inside or before an asm block.  A group of
+   G >= 3 products therefore issues its G multiply-adds, each with its own
+   SGPR pair, before its G carry adds: every carry is read at least two
+   instructions after it was written.  Groups of 1 and 2 fill the gap with
+   s_nop.  FIRST starts c2 from zero (a column's first group). */
+template <int G, bool FIRST>
+MP_HD void macg(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+#if defined(__HIP_DEVICE_COMPILE__)
+    ;
+template <>
+MP_HD void macg<1, true>(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+{
+    uint64_t s0;
+    asm("v_mad_u64_u32 %0, %2, %3, %4, %0\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32 %1, %2, 0, 0, %2"
+        : "+v"(acc), "=v"(c2), "=&s"(s0)
+        : "v"(a[0]), "v"(b[0]));
+}
+template <>
+MP_HD void macg<1, false>(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+{
+    uint64_t s0;
+    asm("v_mad_u64_u32 %0, %2, %3, %4, %0\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32 %1, %2, 0, %1, %2"
+        : "+v"(acc), "+v"(c2), "=&s"(s0)
+        : "v"(a[0]), "v"(b[0]));
+}
+template <>
+MP_HD void macg<2, true>(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+{
+    uint64_t s0, s1;
+    asm("v_mad_u64_u32 %0, %2, %4, %5, %0\n\t"
+        "v_mad_u64_u32 %0, %3, %6, %7, %0\n\t"
+        "s_nop 0\n\t"
+        "v_addc_co_u32 %1, %2, 0, 0, %2\n\t"
+        "v_addc_co_u32 %1, %3, 0, %1, %3"
+        : "+v"(acc), "=v"(c2), "=&s"(s0), "=&s"(s1)
+        : "v"(a[0]), "v"(b[0]), "v"(a[1]), "v"(b[1]));
+}
+template <>
+MP_HD void macg<2, false>(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+{
+    uint64_t s0, s1;
+    asm("v_mad_u64_u32 %0, %2, %4, %5, %0\n\t"
+        "v_mad_u64_u32 %0, %3, %6, %7, %0\n\t"
+        "s_nop 0\n\t"
+        "v_addc_co_u32 %1, %2, 0, %1, %2\n\t"
+        "v_addc_co_u32 %1, %3, 0, %1, %3"
+        : "+v"(acc), "+v"(c2), "=&s"(s0), "=&s"(s1)
+        : "v"(a[0]), "v"(b[0]), "v"(a[1]), "v"(b[1]));
+}
+template <>
+MP_HD void macg<3, true>(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+{
+    uint64_t s0, s1, s2;
+    asm("v_mad_u64_u32 %0, %2, %5, %6, %0\n\t"
+        "v_mad_u64_u32 %0, %3, %7, %8, %0\n\t"
+        "v_mad_u64_u32 %0, %4, %9, %10, %0\n\t"
+        "v_addc_co_u32 %1, %2, 0, 0, %2\n\t"
+        "v_addc_co_u32 %1, %3, 0, %1, %3\n\t"
+        "v_addc_co_u32 %1, %4, 0, %1, %4"
+        : "+v"(acc), "=v"(c2), "=&s"(s0), "=&s"(s1), "=&s"(s2)
+        : "v"(a[0]), "v"(b[0]), "v"(a[1]), "v"(b[1]), "v"(a[2]), "v"(b[2]));
+}
+template <>
+MP_HD void macg<3, false>(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+{
+    uint64_t s0, s1, s2;
+    asm("v_mad_u64_u32 %0, %2, %5, %6, %0\n\t"
+        "v_mad_u64_u32 %0, %3, %7, %8, %0\n\t"
+        "v_mad_u64_u32 %0, %4, %9, %10, %0\n\t"
+        "v_addc_co_u32 %1, %2, 0, %1, %2\n\t"
+        "v_addc_co_u32 %1, %3, 0, %1, %3\n\t"
+        "v_addc_co_u32 %1, %4, 0, %1, %4"
+        : "+v"(acc), "+v"(c2), "=&s"(s0), "=&s"(s1), "=&s"(s2)
+        : "v"(a[0]), "v"(b[0]), "v"(a[1]), "v"(b[1]), "v"(a[2]), "v"(b[2]));
+}
+template <>
+MP_HD void macg<4, true>(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+{
+    uint64_t s0, s1, s2, s3;
+    asm("v_mad_u64_u32 %0, %2, %6, %7, %0\n\t"
+        "v_mad_u64_u32 %0, %3, %8, %9, %0\n\t"
+        "v_mad_u64_u32 %0, %4, %10, %11, %0\n\t"
+        "v_mad_u64_u32 %0, %5, %12, %13, %0\n\t"
+        "v_addc_co_u32 %1, %2, 0, 0, %2\n\t"
+        "v_addc_co_u32 %1, %3, 0, %1, %3\n\t"
+        "v_addc_co_u32 %1, %4, 0, %1, %4\n\t"
+        "v_addc_co_u32 %1, %5, 0, %1, %5"
+        : "+v"(acc), "=v"(c2), "=&s"(s0), "=&s"(s1), "=&s"(s2), "=&s"(s3)
+        : "v"(a[0]), "v"(b[0]), "v"(a[1]), "v"(b[1]), "v"(a[2]), "v"(b[2]), "v"(a[3]), "v"(b[3]));
+}
+template <>
+MP_HD void macg<4, false>(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+{
+    uint64_t s0, s1, s2, s3;
+    asm("v_mad_u64_u32 %0, %2, %6, %7, %0\n\t"
+        "v_mad_u64_u32 %0, %3, %8, %9, %0\n\t"
+        "v_mad_u64_u32 %0, %4, %10, %11, %0\n\t"
+        "v_mad_u64_u32 %0, %5, %12, %13, %0\n\t"
+        "v_addc_co_u32 %1, %2, 0, %1, %2\n\t"
+        "v_addc_co_u32 %1, %3, 0, %1, %3\n\t"
+        "v_addc_co_u32 %1, %4, 0, %1, %4\n\t"
+        "v_addc_co_u32 %1, %5, 0, %1, %5"
+        : "+v"(acc), "+v"(c2), "=&s"(s0), "=&s"(s1), "=&s"(s2), "=&s"(s3)
+        : "v"(a[0]), "v"(b[0]), "v"(a[1]), "v"(b[1]), "v"(a[2]), "v"(b[2]), "v"(a[3]), "v"(b[3]));
+}
+template <>
+MP_HD void macg<5, true>(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+{
+    uint64_t s0, s1, s2, s3, s4;
+    asm("v_mad_u64_u32 %0, %2, %7, %8, %0\n\t"
+        "v_mad_u64_u32 %0, %3, %9, %10, %0\n\t"
+        "v_mad_u64_u32 %0, %4, %11, %12, %0\n\t"
+        "v_mad_u64_u32 %0, %5, %13, %14, %0\n\t"
+        "v_mad_u64_u32 %0, %6, %15, %16, %0\n\t"
+        "v_addc_co_u32 %1, %2, 0, 0, %2\n\t"
+        "v_addc_co_u32 %1, %3, 0, %1, %3\n\t"
+        "v_addc_co_u32 %1, %4, 0, %1, %4\n\t"
+        "v_addc_co_u32 %1, %5, 0, %1, %5\n\t"
+        "v_addc_co_u32 %1, %6, 0, %1, %6"
+        : "+v"(acc), "=v"(c2), "=&s"(s0), "=&s"(s1), "=&s"(s2), "=&s"(s3), "=&s"(s4)
+        : "v"(a[0]), "v"(b[0]), "v"(a[1]), "v"(b[1]), "v"(a[2]), "v"(b[2]), "v"(a[3]), "v"(b[3]), "v"(a[4]), "v"(b[4]));
+}
+template <>
+MP_HD void macg<5, false>(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+{
+    uint64_t s0, s1, s2, s3, s4;
+    asm("v_mad_u64_u32 %0, %2, %7, %8, %0\n\t"
+        "v_mad_u64_u32 %0, %3, %9, %10, %0\n\t"
+        "v_mad_u64_u32 %0, %4, %11, %12, %0\n\t"
+        "v_mad_u64_u32 %0, %5, %13, %14, %0\n\t"
+        "v_mad_u64_u32 %0, %6, %15, %16, %0\n\t"
+        "v_addc_co_u32 %1, %2, 0, %1, %2\n\t"
+        "v_addc_co_u32 %1, %3, 0, %1, %3\n\t"
+        "v_addc_co_u32 %1, %4, 0, %1, %4\n\t"
+        "v_addc_co_u32 %1, %5, 0, %1, %5\n\t"
+        "v_addc_co_u32 %1, %6, 0, %1, %6"
+        : "+v"(acc), "+v"(c2), "=&s"(s0), "=&s"(s1), "=&s"(s2), "=&s"(s3), "=&s"(s4)
+        : "v"(a[0]), "v"(b[0]), "v"(a[1]), "v"(b[1]), "v"(a[2]), "v"(b[2]), "v"(a[3]), "v"(b[3]), "v"(a[4]), "v"(b[4]));
+}
+template <>
+MP_HD void macg<6, true>(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+{
+    uint64_t s0, s1, s2, s3, s4, s5;
+    asm("v_mad_u64_u32 %0, %2, %8, %9, %0\n\t"
+        "v_mad_u64_u32 %0, %3, %10, %11, %0\n\t"
+        "v_mad_u64_u32 %0, %4, %12, %13, %0\n\t"
+        "v_mad_u64_u32 %0, %5, %14, %15, %0\n\t"
+        "v_mad_u64_u32 %0, %6, %16, %17, %0\n\t"
+        "v_mad_u64_u32 %0, %7, %18, %19, %0\n\t"
+        "v_addc_co_u32 %1, %2, 0, 0, %2\n\t"
+        "v_addc_co_u32 %1, %3, 0, %1, %3\n\t"
+        "v_addc_co_u32 %1, %4, 0, %1, %4\n\t"
+        "v_addc_co_u32 %1, %5, 0, %1, %5\n\t"
+        "v_addc_co_u32 %1, %6, 0, %1, %6\n\t"
+        "v_addc_co_u32 %1, %7, 0, %1, %7"
+        : "+v"(acc), "=v"(c2), "=&s"(s0), "=&s"(s1), "=&s"(s2), "=&s"(s3), "=&s"(s4), "=&s"(s5)
+        : "v"(a[0]), "v"(b[0]), "v"(a[1]), "v"(b[1]), "v"(a[2]), "v"(b[2]), "v"(a[3]), "v"(b[3]), "v"(a[4]), "v"(b[4]), "v"(a[5]), "v"(b[5]));
+}
+template <>
+MP_HD void macg<6, false>(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+{
+    uint64_t s0, s1, s2, s3, s4, s5;
+    asm("v_mad_u64_u32 %0, %2, %8, %9, %0\n\t"
+        "v_mad_u64_u32 %0, %3, %10, %11, %0\n\t"
+        "v_mad_u64_u32 %0, %4, %12, %13, %0\n\t"
+        "v_mad_u64_u32 %0, %5, %14, %15, %0\n\t"
+        "v_mad_u64_u32 %0, %6, %16, %17, %0\n\t"
+        "v_mad_u64_u32 %0, %7, %18, %19, %0\n\t"
+        "v_addc_co_u32 %1, %2, 0, %1, %2\n\t"
+        "v_addc_co_u32 %1, %3, 0, %1, %3\n\t"
+        "v_addc_co_u32 %1, %4, 0, %1, %4\n\t"
+        "v_addc_co_u32 %1, %5, 0, %1, %5\n\t"
+        "v_addc_co_u32 %1, %6, 0, %1, %6\n\t"
+        "v_addc_co_u32 %1, %7, 0, %1, %7"
+        : "+v"(acc), "+v"(c2), "=&s"(s0), "=&s"(s1), "=&s"(s2), "=&s"(s3), "=&s"(s4), "=&s"(s5)
+        : "v"(a[0]), "v"(b[0]), "v"(a[1]), "v"(b[1]), "v"(a[2]), "v"(b[2]), "v"(a[3]), "v"(b[3]), "v"(a[4]), "v"(b[4]), "v"(a[5]), "v"(b[5]));
+}
+
+#else
+{
+    if (FIRST)
+        c2 = 0;
+    for (int k = 0; k < G; ++k) {
+        const uint64_t p = (uint64_t)a[k] * b[k];
+        const uint64_t t = acc + p;
+        c2 += (uint32_t)(t < p);
+        acc = t;
+    }
+}
+#endif
+
+/* (c2 : acc) += sum of a[t] * b[t] over t < n, in groups of at most 6
+   (4 while more than 6 are left, so no group but a column's only one is
+   smaller than 3); FIRST: c2 starts from zero */
+template <int n, bool FIRST>
+MP_HD void mac_col(const uint32_t* a, const uint32_t* b, uint64_t& acc, uint32_t& c2)
+{
+    constexpr int g = n > 6 ? 4 : n;
+    macg<g, FIRST>(a, b, acc, c2);
+    if constexpr (n > g)
+        mac_col<n - g, false>(a + g, b + g, acc, c2);
+}
+
+/* one product added to a running column (the squares' diagonal, the GMP
+   path's loops) */
 MP_HD void mac96(uint32_t a, uint32_t b, uint64_t& acc, uint32_t& c2)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
-    uint64_t cc;
-    asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
-        "v_addc_co_u32 %2, %1, 0, %2, %1"
-        : "+v"(acc), "=&s"(cc), "+v"(c2)
-        : "v"(a), "v"(b));
-#else
-    const uint64_t p = (uint64_t)a * b;
-    const uint64_t t = acc + p;
-    c2 += (uint32_t)(t < p);
-    acc = t;
-#endif
+    macg<1, false>(&a, &b, acc, c2);
 }
 
-/* independent accumulators per column.  Splitting a long column into K
-   dependency chains (merged at the column's end) only pays when a SIMD holds
-   a single wave; with two or more resident waves one chain is fastest
-   (measured at N = 32: K = 1, 2, 4 -> 4.7, 3.6, 3.0 G pixel-iter/s). */
-#ifndef MDZG_MAC_CHAINS
-#define MDZG_MAC_CHAINS 1
-#endif
-template <int N>
-constexpr int mac_chains()
-{
-    return MDZG_MAC_CHAINS;
-}
-
-/* (acc, c2) += (acc_b, c2_b) over 96 bits */
-MP_HD void acc96_add(uint64_t& acc, uint32_t& c2, uint64_t accb, uint32_t c2b)
-{
-    uint32_t cc;
-    const uint32_t l0 = add_cc((uint32_t)acc, (uint32_t)accb, 0u, cc);
-    const uint32_t l1 = add_cc((uint32_t)(acc >> 32), (uint32_t)(accb >> 32), cc, cc);
-    c2 = c2 + c2b + cc;
-    acc = ((uint64_t)l1 << 32) | l0;
-}
-
-/* two multiply-accumulates in one asm block (one hazard gap instead of two) */
-MP_HD void mac96x2(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint64_t& acc, uint32_t& c2)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    uint64_t cc;
-    asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
-        "v_addc_co_u32 %2, %1, 0, %2, %1\n\t"
-        "v_mad_u64_u32 %0, %1, %5, %6, %0\n\t"
-        "v_addc_co_u32 %2, %1, 0, %2, %1"
-        : "+v"(acc), "=&s"(cc), "+v"(c2)
-        : "v"(a0), "v"(b0), "v"(a1), "v"(b1));
-#else
-    mac96(a0, b0, acc, c2);
-    mac96(a1, b1, acc, c2);
-#endif
-}
-
-/* first product of a column: the third accumulator word starts from the carry */
+/* the first product of a column: c2 starts from its carry */
 MP_HD void mac96_first(uint32_t a, uint32_t b, uint64_t& acc, uint32_t& c2)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
-    uint64_t cc;
-    asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
-        "v_addc_co_u32 %2, %1, 0, 0, %1"
-        : "+v"(acc), "=&s"(cc), "=v"(c2)
-        : "v"(a), "v"(b));
-#else
-    const uint64_t p = (uint64_t)a * b;
-    const uint64_t t = acc + p;
-    c2 = (uint32_t)(t < p);
-    acc = t;
-#endif
+    macg<1, true>(&a, &b, acc, c2);
+}
+
+/* columns k = K .. 2N-2 of a * b: products a[i] b[k-i] over 0 <= i, k-i < N */
+template <int N, int K>
+MP_HD void mul_cols(const uint32_t* a, const uint32_t* b, uint32_t* H, uint64_t& acc,
+                    uint32_t& st)
+{
+    if constexpr (K < 2 * N - 1) {
+        constexpr int i0 = K < N ? 0 : K - N + 1;
+        constexpr int n = K < N ? K + 1 : 2 * N - 1 - K;
+        uint32_t ca[n], cb[n];
+#pragma unroll
+        for (int t = 0; t < n; ++t) {
+            ca[t] = a[i0 + t];
+            cb[t] = b[K - i0 - t];
+        }
+        uint32_t c2;
+        mac_col<n, true>(ca, cb, acc, c2);
+        if (K < N - 1)
+            st |= (uint32_t)acc;
+        else
+            H[K - (N - 1)] = (uint32_t)acc;
+        acc = (acc >> 32) | ((uint64_t)c2 << 32);
+        mul_cols<N, K + 1>(a, b, H, acc, st);
+    }
 }
 
 /* K0 > 0 leaves out the columns below K0 (a short product, see mul_rn);
@@ -378,54 +564,52 @@ MP_HD void mac96_first(uint32_t a, uint32_t b, uint64_t& acc, uint32_t& c2)
 template <int N, int K0>
 MP_HD void mant_mul(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t& S)
 {
-    constexpr int K = mac_chains<N>();
     uint64_t acc = 0;
-    uint32_t c2 = 0, st = 0;
-#pragma unroll
-    for (int k = K0; k < 2 * N - 1; ++k) {
-        uint64_t xa[K];
-        uint32_t xc[K];
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            xa[q] = q == 0 ? acc : 0;
-            xc[q] = 0;       /* set by the chain's first multiply-accumulate */
-        }
-        int n = 0;
-        int pend = -1;   /* K == 1: a product waiting for its pair */
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const int j = k - i;
-            if (j >= 0 && j < N) {
-                if (n < K) {  /* first product of chain n */
-                    mac96_first(a[i], b[j], xa[n % K], xc[n % K]);
-                } else if (K == 1) {
-                    if (pend < 0) {
-                        pend = i;
-                    } else {
-                        mac96x2(a[pend], b[k - pend], a[i], b[j], xa[0], xc[0]);
-                        pend = -1;
-                    }
-                } else {
-                    mac96(a[i], b[j], xa[n % K], xc[n % K]);
-                }
-                ++n;
-            }
-        }
-        if (K == 1 && pend >= 0)
-            mac96(a[pend], b[k - pend], xa[0], xc[0]);
-#pragma unroll
-        for (int q = 1; q < K; ++q)
-            if (q < n) acc96_add(xa[0], xc[0], xa[q], xc[q]);
-        acc = xa[0];
-        c2 = xc[0];
-        if (k < N - 1)
-            st |= (uint32_t)acc;
-        else
-            H[k - (N - 1)] = (uint32_t)acc;
-        acc = (acc >> 32) | ((uint64_t)c2 << 32);
-    }
+    uint32_t st = 0;
+    mul_cols<N, K0>(a, b, H, acc, st);
     H[N] = (uint32_t)acc;
     S = (uint32_t)(st != 0);
+}
+
+/* columns k = K .. 2N-2 of a * a with each off-diagonal product formed once
+   and the column's off-diagonal sum doubled */
+template <int N, int K>
+MP_HD void sqr_cols(const uint32_t* a, uint32_t* H, uint64_t& acc, uint32_t& st)
+{
+    if constexpr (K < 2 * N - 1) {
+        constexpr int i0 = K < N ? 0 : K - N + 1;
+        constexpr int i1 = (K - 1) / 2;             /* largest i with i < K - i */
+        constexpr int n = i1 >= i0 ? i1 - i0 + 1 : 0;
+        uint32_t c2 = 0;
+        if constexpr (n > 0) {
+            uint32_t ca[n], cb[n];
+#pragma unroll
+            for (int t = 0; t < n; ++t) {
+                ca[t] = a[i0 + t];
+                cb[t] = a[K - i0 - t];
+            }
+            uint64_t o = 0;
+            uint32_t o2;
+            mac_col<n, true>(ca, cb, o, o2);
+            /* acc += 2 * (o2:o) */
+            const uint32_t d0 = (uint32_t)o << 1;
+            const uint32_t d1 = funnel_r((uint32_t)(o >> 32), (uint32_t)o, 31u);
+            const uint32_t d2 = funnel_r(o2, (uint32_t)(o >> 32), 31u);
+            uint32_t cc;
+            const uint32_t l0 = add_cc((uint32_t)acc, d0, 0u, cc);
+            const uint32_t l1 = add_cc((uint32_t)(acc >> 32), d1, cc, cc);
+            c2 = d2 + cc;
+            acc = ((uint64_t)l1 << 32) | l0;
+        }
+        if constexpr ((K & 1) == 0)
+            mac96(a[K >> 1], a[K >> 1], acc, c2);
+        if (K < N - 1)
+            st |= (uint32_t)acc;
+        else
+            H[K - (N - 1)] = (uint32_t)acc;
+        acc = (acc >> 32) | ((uint64_t)c2 << 32);
+        sqr_cols<N, K + 1>(a, H, acc, st);
+    }
 }
 
 /* Square with the off-diagonal products formed once and doubled per column
@@ -433,64 +617,9 @@ MP_HD void mant_mul(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t&
 template <int N, int K0>
 MP_HD void mant_sqr_sym(const uint32_t* a, uint32_t* H, uint32_t& S)
 {
-    constexpr int K = mac_chains<N>();
     uint64_t acc = 0;
-    uint32_t c2 = 0, st = 0;
-#pragma unroll
-    for (int k = K0; k < 2 * N - 1; ++k) {
-        uint64_t xa[K];
-        uint32_t xc[K];
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            xa[q] = 0;
-            xc[q] = 0;
-        }
-        int n = 0;
-        int pend = -1;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const int j = k - i;
-            if (j > i && j < N) {
-                if (n < K) {
-                    mac96_first(a[i], a[j], xa[n % K], xc[n % K]);
-                } else if (K == 1) {
-                    if (pend < 0) {
-                        pend = i;
-                    } else {
-                        mac96x2(a[pend], a[k - pend], a[i], a[j], xa[0], xc[0]);
-                        pend = -1;
-                    }
-                } else {
-                    mac96(a[i], a[j], xa[n % K], xc[n % K]);
-                }
-                ++n;
-            }
-        }
-        if (K == 1 && pend >= 0)
-            mac96(a[pend], a[k - pend], xa[0], xc[0]);
-#pragma unroll
-        for (int q = 1; q < K; ++q)
-            if (q < n) acc96_add(xa[0], xc[0], xa[q], xc[q]);
-        const uint64_t o = xa[0];
-        const uint32_t o2 = xc[0];
-        /* acc += 2 * (o2:o) */
-        const uint32_t d0 = (uint32_t)o << 1;
-        const uint32_t d1 = funnel_r((uint32_t)(o >> 32), (uint32_t)o, 31u);
-        const uint32_t d2 = funnel_r(o2, (uint32_t)(o >> 32), 31u);
-        uint32_t cc;
-        const uint32_t l0 = add_cc((uint32_t)acc, d0, 0u, cc);
-        const uint32_t l1 = add_cc((uint32_t)(acc >> 32), d1, cc, cc);
-        c2 = c2 + d2 + cc;
-        acc = ((uint64_t)l1 << 32) | l0;
-        if ((k & 1) == 0)
-            mac96(a[k >> 1], a[k >> 1], acc, c2);
-        if (k < N - 1)
-            st |= (uint32_t)acc;
-        else
-            H[k - (N - 1)] = (uint32_t)acc;
-        acc = (acc >> 32) | ((uint64_t)c2 << 32);
-        c2 = 0;
-    }
+    uint32_t st = 0;
+    sqr_cols<N, K0>(a, H, acc, st);
     H[N] = (uint32_t)acc;
     S = (uint32_t)(st != 0);
 }

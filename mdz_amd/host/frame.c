@@ -6,7 +6,8 @@
  * fills img->raw_data[line*W .. +W) and returns 1, or 0 once the render was
  * stopped (src/fractal.h:31-44, src/fractal.c:113-116, :236-240).
  *
- * Here the frame is cut into row bands of at least MIN_BAND_PIXELS pixels;
+ * Here the frame is cut into row bands of at least the context's band size
+ * (MDZG_OPT_BAND_PIXELS, 2^18 pixels unless tuned) in pixels;
  * bands are submitted to the device in row order, a few at a time, as the
  * workers ask for their rows.  A call for row `line` makes sure its band (and
  * the next ones) are in flight, waits for its band, copies the row into the
@@ -19,15 +20,13 @@
 
 #include "mdzg_host.h"
 
-#define MIN_BAND_PIXELS (1 << 18)
-#define INFLIGHT 3
-
 struct mdzg_frame {
     mdzg_ctx* ctx;
     mdzg_view view;
     char* strs[6];
     int32_t* raw;
     int band_rows, nbands, next_submit, stopped, err;
+    int inflight;                 /* bands kept in flight (MDZG_OPT_BAND_INFLIGHT) */
     mdzg_job** jobs;
     uint32_t** bufs;
     char* band_done;
@@ -76,7 +75,10 @@ int mdzg_frame_open(mdzg_ctx* ctx, const mdzg_view* view, int32_t* raw, mdzg_fra
     f->view.julia_im = f->strs[5];
     f->raw = raw;
     const int W = view->real_width, H = view->real_height;
-    f->band_rows = (MIN_BAND_PIXELS + W - 1) / W;
+    f->band_rows = (int)((ctx->band_pixels + W - 1) / W);
+    if (f->band_rows < 1)
+        f->band_rows = 1;
+    f->inflight = ctx->band_inflight;
     if (f->band_rows > H)
         f->band_rows = H;
     f->nbands = (H + f->band_rows - 1) / f->band_rows;
@@ -141,7 +143,7 @@ int mdzg_frame_calculate_line(mdzg_frame* f, int line)
         pthread_mutex_unlock(&f->lock);
         return 0;
     }
-    int rc = submit_upto(f, b + INFLIGHT - 1);
+    int rc = submit_upto(f, b + f->inflight - 1);
     if (rc != MDZG_OK) {
         f->err = rc;
         pthread_mutex_unlock(&f->lock);

@@ -23,6 +23,24 @@ int mdzg_dev_launch_persist(int kid, const mdzg_dtile* d_tiles, int ntiles, uint
                             unsigned long long* d_counter, const int* d_cancel, int grid,
                             int chunk, hipStream_t st);
 int mdzg_dev_occupancy_slice(int kid);
+/* launch nodes (launch.hip, k_colour.hip) and their graph replay (graph.hip) */
+int mdzg_dev_node_fill(mdzg_gnode* n, uint32_t* p, uint64_t count, uint32_t v);
+int mdzg_dev_node_persist(mdzg_gnode* n, int kid, const mdzg_dtile* d_tiles, int ntiles,
+                          uint64_t total, unsigned long long* d_counter, const int* d_cancel,
+                          int grid, int chunk);
+int mdzg_dev_node_slice(mdzg_gnode* n, int kid, const mdzg_dtile* d_tiles, int ntiles,
+                        uint64_t total, unsigned long long* d_counter, int grid, int budget);
+int mdzg_dev_node_colour(mdzg_gnode* node, const uint32_t* counts, int real_w, int user_w,
+                         int user_h, int aa, const uint32_t* palette, int n, int offset,
+                         double sc, int ip, uint32_t depth, uint32_t* rgb,
+                         unsigned long long* iters, int max_blocks);
+int mdzg_dev_node_launch(const mdzg_gnode* n, hipStream_t st);
+typedef struct mdzg_gcache mdzg_gcache;
+mdzg_gcache* mdzg_gcache_create(void);
+void mdzg_gcache_destroy(mdzg_gcache* gc);
+void mdzg_gcache_counts(mdzg_gcache* gc, uint64_t* hits, uint64_t* builds);
+int mdzg_graph_run(mdzg_gcache* gc, const mdzg_gnode* root, const mdzg_gnode* par, int npar,
+                   const mdzg_gnode* tail, hipStream_t st);
 int mdzg_dev_launch_slice(int kid, const mdzg_dtile* d_tiles, int ntiles, uint64_t total,
                           unsigned long long* d_counter, int grid, int budget, hipStream_t st);
 
@@ -114,6 +132,11 @@ struct mdzg_ctx {
     void* d_scratch;
     void* h_scratch;
     hipEvent_t na_ev0, na_ev1;
+    /* options: graph replay, the line adapter's bands */
+    int graphs;
+    long long band_pixels;
+    int band_inflight;
+    mdzg_gcache* gc;            /* graphs of persistent batches */
 };
 
 /* noalloc.hip: device address of the static scratch array of the current
@@ -121,5 +144,12 @@ struct mdzg_ctx {
 int mdzg_dev_scratch(int device, void** ptr, size_t* bytes);
 
 double mdzg_now_ms(void);
+
+/* the line adapter's band defaults (MDZG_OPT_BAND_PIXELS / _INFLIGHT) */
+#define MDZG_DEFAULT_BAND_PIXELS (1 << 18)
+#define MDZG_DEFAULT_BAND_INFLIGHT 3
+
+/* directory of the loaded libmdzg.so (tune.c); 0 on success */
+int mdzg_lib_dir(char* buf, size_t cap);
 
 #endif

@@ -161,8 +161,27 @@ int mdzg_create(int device, mdzg_ctx** out)
             c->occ[k] = 1;
         c->chunk[k] = default_chunk(k);
     }
+    c->graphs = 1;
+    c->band_pixels = MDZG_DEFAULT_BAND_PIXELS;
+    c->band_inflight = MDZG_DEFAULT_BAND_INFLIGHT;
+    c->gc = mdzg_gcache_create();
+    if (!c->gc)
+        goto fail;
     pthread_mutex_init(&c->lock, NULL);
     pthread_mutex_init(&f->rt.cache_lock, NULL);
+    /* the tuned table: MDZG_TUNE_TABLE, else tune_gfx950.txt beside the library */
+    {
+        const char* env = getenv("MDZG_TUNE_TABLE");
+        char path[4096];
+        if (env && *env) {
+            snprintf(path, sizeof path, "%s", env);
+            mdzg_tune_load(c, path);
+        } else if (mdzg_lib_dir(path, sizeof path) == 0) {
+            size_t n = strlen(path);
+            snprintf(path + n, sizeof path - n, "/tune_gfx950.txt");
+            mdzg_tune_load(c, path);
+        }
+    }
     *out = c;
     return MDZG_OK;
 fail:
@@ -187,6 +206,7 @@ void mdzg_destroy(mdzg_ctx* c)
     if (c->stream) hipStreamDestroy(c->stream);
     if (c->ctl) hipStreamDestroy(c->ctl);
     if (c->d_cancel_zero) hipFree(c->d_cancel_zero);
+    mdzg_gcache_destroy(c->gc);
     if (c->na_ev0) hipEventDestroy(c->na_ev0);
     if (c->na_ev1) hipEventDestroy(c->na_ev1);
     free(c->h_scratch);
@@ -324,7 +344,7 @@ static void job_free_mem(mdzg_job* j)
 }
 
 static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int mode,
-                          const mdzg_image_req* img, mdzg_job** out)
+                          const mdzg_image_req* img, int sync, mdzg_job** out)
 {
     if (!c || !tiles || ntiles < 1 || !out || (mode != MDZG_BATCH_PERSISTENT && mode != MDZG_BATCH_PER_TILE))
         return MDZG_ERR_ARG;
@@ -491,24 +511,19 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
             rc = MDZG_ERR_DEVICE;
             goto done;
         }
-        /* counts start at the UINT_MAX sentinel of src/image_info.c:214-215, so a
-           cancelled job leaves the pixels it never finished recognisable */
-        if (hipMemsetD32Async((hipDeviceptr_t)(D + j->off_out), 0xffffffffu, j->out_bytes / 4,
-                              j->stream) != hipSuccess) {
+        /* the counts of an asynchronous job, which can be cancelled, start at
+           the UINT_MAX sentinel of src/image_info.c:214-215, so that the pixels
+           it never finished stay recognisable: a fill kernel, the root of the
+           launch plan.  A synchronous render cannot be cancelled and its
+           kernels write every count, so it needs no fill. */
+        mdzg_gnode fillnode, tail;
+        mdzg_gnode* fill = sync ? NULL : &fillnode;
+        mdzg_gnode par[MDZG_NKERNELS];
+        int npar = 0;
+        if (fill && mdzg_dev_node_fill(fill, (uint32_t*)(D + j->off_out), j->out_bytes / 4,
+                                       0xffffffffu) != 0) {
             rc = MDZG_ERR_DEVICE;
             goto done;
-        }
-        hipEventRecord(j->ev0, j->stream);
-        /* persistent mode: each arithmetic class drains its own queue; the
-           class launches fork onto separate streams so their tails overlap,
-           and join back before the counts are copied out */
-        const int fork = (mode == MDZG_BATCH_PERSISTENT && j->nkid > 1);
-        if (fork) {
-            if (hipEventCreateWithFlags(&j->ev_fork, hipEventDisableTiming) != hipSuccess) {
-                rc = MDZG_ERR_DEVICE;
-                goto done;
-            }
-            hipEventRecord(j->ev_fork, j->stream);
         }
         int first = 0;
         for (int k = 0; k < j->nkid; ++k) {
@@ -521,58 +536,115 @@ static int mdzg_job_start(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int m
                 ++cnt;
             }
             if (mode == MDZG_BATCH_PERSISTENT) {
+                /* each arithmetic class drains its own queue; the class
+                   launches run concurrently after the fill */
+                uint64_t cap = (uint64_t)c->ncu * (uint64_t)c->occ[kid];
+                uint64_t need = (total + 255) / 256;
+                int grid = (int)(need < cap ? need : cap);
+                if (mdzg_dev_node_persist(&par[npar++], kid, d_desc + first, cnt, total, d_ctr + k,
+                                          d_cancel, grid, c->chunk[kid]) != 0) {
+                    rc = MDZG_ERR_DEVICE;
+                    goto done;
+                }
+            }
+            first += cnt;
+        }
+        const int has_tail = img != NULL;
+        if (img) {
+            /* one tile, the whole frame: colour it where it lies */
+            const mdzg_palette* P = img->pal;
+            if (mdzg_dev_node_colour(&tail, (const uint32_t*)(D + j->out_off[0]), tiles[0].ncols,
+                                     img->user_w, img->user_h, img->aa,
+                                     (const uint32_t*)(D + j->off_pal), P->n, P->offset,
+                                     P->colour_scale, P->interpolate,
+                                     (uint32_t)tiles[0].view.depth, (uint32_t*)(D + j->off_rgb),
+                                     (unsigned long long*)(D + j->off_iters),
+                                     MDZG_COLOUR_BLOCKS_PER_CU * c->ncu) != 0) {
+                rc = MDZG_ERR_DEVICE;
+                goto done;
+            }
+        }
+        hipEventRecord(j->ev0, j->stream);
+        if (mode == MDZG_BATCH_PERSISTENT && c->graphs) {
+            /* the whole plan as one replay of a cached graph */
+            if (mdzg_graph_run(c->gc, fill, par, npar, has_tail ? &tail : NULL, j->stream) != 0) {
+                rc = MDZG_ERR_DEVICE;
+                goto done;
+            }
+            j->launches += (fill != NULL) + npar + has_tail;
+        } else if (mode == MDZG_BATCH_PERSISTENT) {
+            /* launched one by one: the class launches fork onto separate
+               streams so their tails overlap, and join back */
+            if (fill && mdzg_dev_node_launch(fill, j->stream) != 0) {
+                rc = MDZG_ERR_DEVICE;
+                goto done;
+            }
+            const int fork = npar > 1;
+            if (fork) {
+                if (hipEventCreateWithFlags(&j->ev_fork, hipEventDisableTiming) != hipSuccess) {
+                    rc = MDZG_ERR_DEVICE;
+                    goto done;
+                }
+                hipEventRecord(j->ev_fork, j->stream);
+            }
+            for (int k = 0; k < npar; ++k) {
                 hipStream_t st = j->stream;
                 if (fork && k > 0) {
                     st = side_stream(c, j->stream, k);
                     hipStreamWaitEvent(st, j->ev_fork, 0);
                 }
-                uint64_t cap = (uint64_t)c->ncu * (uint64_t)c->occ[kid];
-                uint64_t need = (total + 255) / 256;
-                int grid = (int)(need < cap ? need : cap);
-                int chunk = c->chunk[kid];
-                if (mdzg_dev_launch_persist(kid, d_desc + first, cnt, total, d_ctr + k, d_cancel,
-                                            grid, chunk, st) != 0) {
+                if (mdzg_dev_node_launch(&par[k], st) != 0) {
                     rc = MDZG_ERR_DEVICE;
                     goto done;
                 }
                 if (fork && k > 0) {
-                    if (hipEventCreateWithFlags(&j->ev_cls[k], hipEventDisableTiming) != hipSuccess) {
+                    if (hipEventCreateWithFlags(&j->ev_cls[k], hipEventDisableTiming) !=
+                        hipSuccess) {
                         rc = MDZG_ERR_DEVICE;
                         goto done;
                     }
                     hipEventRecord(j->ev_cls[k], st);
                 }
+            }
+            if (fork)
+                for (int k = 1; k < npar; ++k)
+                    hipStreamWaitEvent(j->stream, j->ev_cls[k], 0);
+            if (has_tail && mdzg_dev_node_launch(&tail, j->stream) != 0) {
+                rc = MDZG_ERR_DEVICE;
+                goto done;
+            }
+            j->launches += (fill != NULL) + npar + has_tail;
+        } else {
+            /* the per-tile baseline: one plain launch per tile, in order */
+            if (fill) {
+                if (mdzg_dev_node_launch(fill, j->stream) != 0) {
+                    rc = MDZG_ERR_DEVICE;
+                    goto done;
+                }
                 j->launches++;
-            } else {
+            }
+            int f0 = 0;
+            for (int k = 0; k < j->nkid; ++k) {
+                const int kid = j->kids[k];
+                int cnt = 0;
+                while (f0 + cnt < ntiles && pv[order[f0 + cnt]].kid == kid)
+                    ++cnt;
                 for (int n = 0; n < cnt; ++n) {
-                    const mdzg_tile* t = &tiles[order[first + n]];
+                    const mdzg_tile* t = &tiles[order[f0 + n]];
                     uint32_t npix = (uint32_t)t->ncols * (uint32_t)t->nlines;
-                    if (mdzg_dev_launch_tile(kid, d_desc + first + n, npix, j->stream) != 0) {
+                    if (mdzg_dev_launch_tile(kid, d_desc + f0 + n, npix, j->stream) != 0) {
                         rc = MDZG_ERR_DEVICE;
                         goto done;
                     }
                     j->launches++;
                 }
+                f0 += cnt;
             }
-            first += cnt;
-        }
-        if (fork)
-            for (int k = 1; k < j->nkid; ++k)
-                hipStreamWaitEvent(j->stream, j->ev_cls[k], 0);
-        if (img) {
-            /* one tile, the whole frame: colour it where it lies */
-            const mdzg_palette* P = img->pal;
-            if (mdzg_dev_launch_colour((const uint32_t*)(D + j->out_off[0]), tiles[0].ncols,
-                                       img->user_w, img->user_h, img->aa,
-                                       (const uint32_t*)(D + j->off_pal), P->n, P->offset,
-                                       P->colour_scale, P->interpolate,
-                                       (uint32_t)tiles[0].view.depth, (uint32_t*)(D + j->off_rgb),
-                                       (unsigned long long*)(D + j->off_iters),
-                                       MDZG_COLOUR_BLOCKS_PER_CU * c->ncu, j->stream) != 0) {
+            if (has_tail && mdzg_dev_node_launch(&tail, j->stream) != 0) {
                 rc = MDZG_ERR_DEVICE;
                 goto done;
             }
-            j->launches++;
+            j->launches += has_tail;
         }
         hipEventRecord(j->ev1, j->stream);
         if (img) {
@@ -600,8 +672,13 @@ done:
     free(pv);
     free(order);
     if (rc != MDZG_OK) {
-        if (j->launched || j->stream)
-            hipStreamSynchronize(j->stream);
+        /* nothing the job launched may still use its memory: wait for its
+           stream and for the side streams its class launches forked onto */
+        if (j->launched || j->stream) {
+            ctx_full* f = (ctx_full*)c;
+            for (int i = 0; i < NSTREAMS; ++i)
+                hipStreamSynchronize(f->rt.streams[i]);
+        }
         job_free_mem(j);
         return rc;
     }
@@ -611,7 +688,7 @@ done:
 
 int mdzg_submit(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, mdzg_job** out)
 {
-    int rc = mdzg_job_start(c, tiles, ntiles, MDZG_BATCH_PERSISTENT, NULL, out);
+    int rc = mdzg_job_start(c, tiles, ntiles, MDZG_BATCH_PERSISTENT, NULL, 0, out);
     if (rc == MDZG_OK)
         (*out)->t_start = mdzg_now_ms();
     return rc;
@@ -620,7 +697,7 @@ int mdzg_submit(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, mdzg_job** out)
 int mdzg_rt_submit_image(mdzg_ctx* c, const mdzg_tile* tile, const mdzg_image_req* img,
                          mdzg_job** out)
 {
-    int rc = mdzg_job_start(c, tile, 1, MDZG_BATCH_PERSISTENT, img, out);
+    int rc = mdzg_job_start(c, tile, 1, MDZG_BATCH_PERSISTENT, img, 0, out);
     if (rc == MDZG_OK)
         (*out)->t_start = mdzg_now_ms();
     return rc;
@@ -731,7 +808,7 @@ int mdzg_render_batch(mdzg_ctx* c, const mdzg_tile* tiles, int ntiles, int mode,
 {
     double t0 = mdzg_now_ms();
     mdzg_job* j = NULL;
-    int rc = mdzg_job_start(c, tiles, ntiles, mode, NULL, &j);
+    int rc = mdzg_job_start(c, tiles, ntiles, mode, NULL, 1, &j);
     if (rc != MDZG_OK)
         return rc;
     j->t_start = t0;
@@ -819,7 +896,7 @@ int mdzg_rt_render_image(mdzg_ctx* c, const mdzg_tile* tile, const mdzg_image_re
 {
     double t0 = mdzg_now_ms();
     mdzg_job* j = NULL;
-    int rc = mdzg_job_start(c, tile, 1, MDZG_BATCH_PERSISTENT, img, &j);
+    int rc = mdzg_job_start(c, tile, 1, MDZG_BATCH_PERSISTENT, img, 1, &j);
     if (rc != MDZG_OK)
         return rc;
     j->t_start = t0;

@@ -107,6 +107,10 @@ struct mdzg_server {
     long long budget_pix[MAX_KID];
     long long act_pix[MAX_KID];
     int slice_iters[MAX_KID];
+    /* tests: the id of a request whose staging fails as a device error would
+       (environment variable MDZG_FAILPOINT_STAGE read at mdzg_server_create;
+       0 = none) */
+    uint64_t fail_stage_id;
 
     mdzg_serve_stats st;
 };
@@ -222,6 +226,12 @@ static double count_iters(const uint32_t* c, size_t n, double depth)
 }
 
 /* ---- batched mode: one slice ------------------------------------------------ */
+/* Stage the newly admitted requests.  A request whose staging fails gets
+   the error as its status and takes no part in the slices; the others go
+   on.  The return value is an error only when the staging buffer cannot be
+   had, which fails every new request. */
+static int stage_one(mdzg_server* s, sreq* r, size_t* hoff, hipStream_t st);
+
 static int stage_new(mdzg_server* s, sreq** nw, int nnew, hipStream_t st)
 {
     /* start values go through the pinned staging buffer */
@@ -247,8 +257,24 @@ static int stage_new(mdzg_server* s, sreq** nw, int nnew, hipStream_t st)
     }
     size_t hoff = 0;
     for (int i = 0; i < nnew; ++i) {
-        sreq* r = nw[i];
+        const int rc = stage_one(s, nw[i], &hoff, st);
+        if (rc != MDZG_OK)
+            nw[i]->status = rc;
+        if (nw[i]->parsed) {
+            mdzg_view_clear(&nw[i]->pv);
+            nw[i]->parsed = 0;
+        }
+    }
+    return MDZG_OK;
+}
+
+static int stage_one(mdzg_server* s, sreq* r, size_t* hoffp, hipStream_t st)
+{
+    size_t hoff = *hoffp;
+    {
         const mdzg_tile* t = &r->tile;
+        if (r->id == s->fail_stage_id)
+            return MDZG_ERR_DEVICE;
         char* base = s->d_arena + r->off;
         size_t o = 0;
         if (r->pv.vs) {
@@ -259,8 +285,10 @@ static int stage_new(mdzg_server* s, sreq** nw, int nnew, hipStream_t st)
             uint32_t* ys = (uint32_t*)(H + vsb * (size_t)t->ncols);
             uint32_t* jc = (uint32_t*)(H + vsb * ((size_t)t->ncols + (size_t)t->nlines));
             int rc = mdzg_coords_fill(&r->pv, t->line0, t->nlines, t->col0, t->ncols, xs, ys, jc);
-            if (rc != MDZG_OK)
+            if (rc != MDZG_OK) {
+                *hoffp = hoff;
                 return rc;
+            }
             r->d_xs = (uint32_t*)base;
             r->d_ys = (uint32_t*)(base + vsb * (size_t)t->ncols);
             r->d_jc = (uint32_t*)(base + vsb * ((size_t)t->ncols + (size_t)t->nlines));
@@ -295,11 +323,12 @@ static int stage_new(mdzg_server* s, sreq** nw, int nnew, hipStream_t st)
         if (hipMemsetD32Async((hipDeviceptr_t)r->d_out, (int)SENT, r->npix, st) != hipSuccess ||
             hipMemsetD32Async((hipDeviceptr_t)r->d_wzs, 0, r->npix, st) != hipSuccess ||
             hipMemsetD32Async((hipDeviceptr_t)(s->d_remaining + r->slot), (int)r->npix, 1, st) !=
-                hipSuccess)
+                hipSuccess) {
+            *hoffp = hoff;
             return MDZG_ERR_DEVICE;
-        mdzg_view_clear(&r->pv);
-        r->parsed = 0;
+        }
     }
+    *hoffp = hoff;
     return MDZG_OK;
 }
 
@@ -326,7 +355,7 @@ static int run_slice(mdzg_server* s)
         total[k] = 0;
         for (int i = 0; i < s->nact; ++i) {
             sreq* r = s->act[i];
-            if (r->pv.kid != k)
+            if (r->pv.kid != k || r->status < 0)
                 continue;
             mdzg_dtile* d = &s->h_desc[n++];
             memset(d, 0, sizeof *d);
@@ -358,30 +387,47 @@ static int run_slice(mdzg_server* s)
             cnt[k]++;
         }
     }
+    if (n == 0)
+        return MDZG_OK;          /* every active request failed its staging */
     if (hipMemcpyAsync(s->d_desc, s->h_desc, sizeof(mdzg_dtile) * (size_t)n,
-                       hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemsetD32Async((hipDeviceptr_t)s->d_ctr, 0, 2 * MAX_KID, st) != hipSuccess)
+                       hipMemcpyHostToDevice, st) != hipSuccess)
         return MDZG_ERR_DEVICE;
-    hipEventRecord(s->ev_fork, st);
+    /* the slice: the queue counters reset by a fill kernel, then one slice
+       launch per class, concurrently -- one graph replay, or launched one by
+       one with a fork and join over the server's streams */
+    mdzg_gnode reset, par[MAX_KID];
     int nk = 0;
+    if (mdzg_dev_node_fill(&reset, (uint32_t*)s->d_ctr, 2 * MAX_KID, 0u) != 0)
+        return MDZG_ERR_DEVICE;
     for (int k = 0; k < MAX_KID; ++k) {
         if (!cnt[k])
             continue;
-        hipStream_t ks = s->streams[nk];
-        if (nk > 0)
-            hipStreamWaitEvent(ks, s->ev_fork, 0);
         const uint64_t cap = (uint64_t)c->ncu * (uint64_t)s->occ_slice[k];
         const uint64_t want = (total[k] + 255) / 256;
         int grid = (int)(want < cap ? want : cap);
         if (grid < 1) grid = 1;
-        if (mdzg_dev_launch_slice(k, s->d_desc + first[k], cnt[k], total[k], s->d_ctr + k, grid,
-                                  s->slice_iters[k], ks) != 0)
+        if (mdzg_dev_node_slice(&par[nk++], k, s->d_desc + first[k], cnt[k], total[k],
+                                s->d_ctr + k, grid, s->slice_iters[k]) != 0)
             return MDZG_ERR_DEVICE;
-        if (nk > 0) {
-            hipEventRecord(s->ev_join[nk], ks);
-            hipStreamWaitEvent(st, s->ev_join[nk], 0);
+    }
+    if (c->graphs) {
+        if (mdzg_graph_run(c->gc, &reset, par, nk, NULL, st) != 0)
+            return MDZG_ERR_DEVICE;
+    } else {
+        if (mdzg_dev_node_launch(&reset, st) != 0)
+            return MDZG_ERR_DEVICE;
+        hipEventRecord(s->ev_fork, st);
+        for (int q = 0; q < nk; ++q) {
+            hipStream_t ks = s->streams[q];
+            if (q > 0)
+                hipStreamWaitEvent(ks, s->ev_fork, 0);
+            if (mdzg_dev_node_launch(&par[q], ks) != 0)
+                return MDZG_ERR_DEVICE;
+            if (q > 0) {
+                hipEventRecord(s->ev_join[q], ks);
+                hipStreamWaitEvent(st, s->ev_join[q], 0);
+            }
         }
-        nk++;
     }
     if (hipMemcpyAsync(s->h_remaining, s->d_remaining, sizeof(unsigned int) * (size_t)s->nslots,
                        hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -477,19 +523,35 @@ static void* sched_batched(void* arg)
         int rc = MDZG_OK;
         if (nnew)
             rc = stage_new(s, fresh, nnew, s->streams[0]);
-        if (rc == MDZG_OK && s->nact)
+        if (rc != MDZG_OK)
+            for (int i = 0; i < nnew; ++i)
+                fresh[i]->status = rc;   /* no staging buffer: every new request fails */
+        rc = MDZG_OK;
+        if (s->nact)
             rc = run_slice(s);
+        /* a failed request's memory goes back to the arena only after
+           everything launched on the server's streams has finished with it */
+        int failed = rc != MDZG_OK;
+        for (int i = 0; i < s->nact && !failed; ++i)
+            failed = s->act[i]->status < 0;
+        if (failed)
+            for (int k = 0; k < MAX_KID; ++k)
+                if (s->streams[k])
+                    hipStreamSynchronize(s->streams[k]);
 
-        /* retire finished requests */
+        /* retire finished requests; a failed slice fails every active
+           request, a failed staging only its own */
         pthread_mutex_lock(&s->lock);
         for (int i = 0; i < s->nact;) {
             sreq* r = s->act[i];
-            const int done = (rc != MDZG_OK) || s->h_remaining[r->slot] == 0;
+            const int done = (rc != MDZG_OK) || r->status < 0 || s->h_remaining[r->slot] == 0;
             if (!done) {
                 ++i;
                 continue;
             }
-            if (rc == MDZG_OK) {
+            if (r->status < 0) {
+                /* its staging failed: the status already holds the error */
+            } else if (rc == MDZG_OK) {
                 pthread_mutex_unlock(&s->lock);
                 const int rr = r->image ? retire_image(s, r) : retire_counts(r);
                 pthread_mutex_lock(&s->lock);
@@ -596,6 +658,10 @@ int mdzg_server_create(mdzg_ctx* ctx, const mdzg_server_cfg* cfg, mdzg_server** 
     s->ctx = ctx;
     if (cfg)
         s->cfg = *cfg;
+    {
+        const char* fp = getenv("MDZG_FAILPOINT_STAGE");
+        s->fail_stage_id = (fp && *fp) ? strtoull(fp, NULL, 10) : 0;
+    }
     if (s->cfg.mode != MDZG_SERVE_BATCHED && s->cfg.mode != MDZG_SERVE_PER_REQUEST) {
         free(s);
         return MDZG_ERR_ARG;

@@ -214,10 +214,16 @@ int orc_use_reference(const char* so_path)
         memset(ref_gmp, 0, sizeof ref_gmp);
         return 0;
     }
-    if (!ref_handle)
-        ref_handle = dlopen(so_path, RTLD_NOW | RTLD_LOCAL);
-    if (!ref_handle)
-        return -1;
+    /* one handle per build of the loops (the -O0 and -O2 libraries); a
+       handle stays open until the process ends */
+    static char ref_loaded[4096];
+    if (!ref_handle || strcmp(ref_loaded, so_path) != 0) {
+        void* h = dlopen(so_path, RTLD_NOW | RTLD_LOCAL);
+        if (!h)
+            return -1;
+        ref_handle = h;
+        snprintf(ref_loaded, sizeof ref_loaded, "%s", so_path);
+    }
     for (int i = 0; i < 4; ++i) {
         ref_ld[i] = (ld_fn)dlsym(ref_handle, ld_names[i]);
         ref_mp[i] = (mp_fn)dlsym(ref_handle, mp_names[i]);

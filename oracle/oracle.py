@@ -76,12 +76,14 @@ def lib():
     return _lib
 
 
-def use_reference(on=True):
-    """Switch the ld/MPFR loops to the compiled reference loops (oracle/_ref)."""
+def use_reference(on=True, build="O0"):
+    """Switch the ld/MPFR/GMP loops to the compiled reference loops (oracle/_ref):
+    build "O0" as the reference builds them (-ggdb), "O2" optimised."""
     if on:
-        if not os.path.exists(REF_PATH):
-            raise RuntimeError("oracle/_ref/libfrac_ref.so not built")
-        rc = lib().orc_use_reference(REF_PATH.encode())
+        path = REF_PATH if build == "O0" else REF_PATH.replace(".so", "_O2.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built")
+        rc = lib().orc_use_reference(path.encode())
     else:
         rc = lib().orc_use_reference(None)
     if rc != 0:

@@ -146,3 +146,40 @@ def test_c_program_renders_a_scene_to_png(ctx, tmp_path):
     w, h, img = png_decode(str(out))
     assert (w, h) == (rgb.shape[1], rgb.shape[0])
     assert np.array_equal(img, _rgb_bytes(rgb))
+
+
+def test_c_line_shim_under_a_row_claiming_pool(ctx, golden, tmp_path):
+    """mdz_amd/shim/fractal_gpu.c, the int (*)(image_info*, int) callback MDZ
+    installs (src/image_info.c:242-248), compiled as C against test-side
+    image_info declarations and called by 16 pthreads that claim rows as
+    src/render_threads.c:342-393 does: a full render equals the device
+    render of the same view; a second render in the same process, stopped
+    after 40 claims, leaves every row either complete and equal or at the
+    UINT_MAX sentinel, as MDZ's stop does (src/image_info.c:214-215)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "tests", "capi", "shim_pool")
+    if not os.path.exists(exe):
+        from tests.capi import build as B
+        B.build()
+    meta = next(m for m, _ in golden if m["name"] == "deep_mpfr128_f2")
+    vd = meta["view"]
+    cx, cy = vd["rect"][0], vd["rect"][2]
+    W, H = 640, 1536                       # four bands of the adapter's default size
+    rect = M.rect_from_center(cx, cy, "1e-24", 128, W, H)
+    v = M.make_view(rect, precision=128, depth=5000, width=W, height=H, fractal=2)
+    want, _ = ctx.render(v)
+    assert len(np.unique(want)) > 50
+    out = tmp_path / "maps.bin"
+    print("shim_pool 640x1536 p=128 celtic", flush=True)
+    r = subprocess.run([exe, "0", "2", str(W), str(H), "128", "5000", *rect, "16", "2", "40",
+                        str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    done1, done2 = (int(x) for x in r.stdout.split()[1::2])
+    maps = np.fromfile(str(out), dtype=np.uint32).reshape(2, H, W)
+    assert done1 == H and np.array_equal(maps[0], want)
+    rows = maps[1]
+    full = (rows == want).all(axis=1)
+    blank = (rows == 0xFFFFFFFF).all(axis=1)
+    assert (full | blank).all()
+    assert blank.sum() > 0 and full.sum() == done2 <= 2 * 40

@@ -154,3 +154,50 @@ def test_server_image_request_arguments(ctx):
         bad.n = 300
         with pytest.raises(M.MdzgError):
             srv.submit_image(v, 1, bad, (0, 4), (0, 4))
+
+
+@pytest.mark.parametrize("graphs", [0, 1])
+def test_server_slices_with_and_without_graphs(ctx, graphs):
+    """Slices replayed as cached HIP graphs (the default) and launched one by
+    one give the standalone counts; several replays of each graph shape run
+    (slice_iters 7: many slices per request)."""
+    reqs = _mix()
+    alone = [ctx.render(v, lines=l, cols=c)[0] for v, l, c in reqs]
+    ctx.option_set(M.OPT_GRAPHS, graphs)
+    try:
+        with M.Server(ctx, slice_iters=7) as srv:
+            ids = [srv.submit(v, l, c) for v, l, c in reqs]
+            outs = [srv.wait(i) for i in ids]
+            assert srv.stats()["slices"] > 100
+    finally:
+        ctx.option_set(M.OPT_GRAPHS, 1)
+    for (out, info), ref in zip(outs, alone):
+        assert info["status"] == 2
+        assert _eq(out, ref)
+
+
+def test_server_staging_failure_fails_only_that_request(ctx, monkeypatch):
+    """A request whose staging fails (forced from the host by the stage
+    failpoint) retires with the error; the requests around it, staged in the
+    same round and resident across its retirement, finish with their
+    standalone counts."""
+    reqs = _mix()
+    alone = [ctx.render(v, lines=l, cols=c)[0] for v, l, c in reqs]
+    monkeypatch.setenv("MDZG_FAILPOINT_STAGE", "3")      # the third request of this server
+    with M.Server(ctx, slice_iters=7) as srv:
+        ids = [srv.submit(v, l, c) for v, l, c in reqs]
+        monkeypatch.delenv("MDZG_FAILPOINT_STAGE")
+        infos = {}
+        outs = {}
+        for i in ids:
+            try:
+                outs[i], infos[i] = srv.wait(i)
+            except M.MdzgError as e:
+                infos[i] = e.code
+    assert ids[2] == 3
+    assert infos[3] == -3
+    for i, ref in zip(ids, alone):
+        if i == 3:
+            continue
+        assert infos[i]["status"] == 2, i
+        assert _eq(outs[i], ref), i
