@@ -257,11 +257,14 @@ int mdzg_tune_get(mdzg_ctx* ctx, int kernel_id, int* blocks_per_cu, int* chunk,
                   int* max_blocks_per_cu);
 
 /* ---- options (no counterpart in the reference) --------------------------
-   MDZG_OPT_GRAPHS        1 (default): a persistent batch (its sentinel fill,
-                          its concurrent class launches and an image job's
-                          colouring pass) and each serving slice (counter
-                          reset and class launches) run as one replay of a
-                          cached HIP graph; 0: launched one by one on streams.
+   MDZG_OPT_GRAPHS        1 (default): a persistent batch of two or more
+                          arithmetic classes (its sentinel fill, its
+                          concurrent class launches and an image job's
+                          colouring pass) and each serving slice of two or
+                          more classes (counter reset and class launches) run
+                          as one replay of a cached HIP graph; plans of one
+                          class are launched directly.  0: every plan is
+                          launched one by one, the classes forked onto streams.
    MDZG_OPT_BAND_PIXELS   the line adapter's band size in pixels (a band is
                           whole rows, at least this many pixels); 1..2^30.
    MDZG_OPT_BAND_INFLIGHT bands the line adapter keeps in flight; 1..16.

@@ -137,6 +137,12 @@ struct mdzg_ctx {
     long long band_pixels;
     int band_inflight;
     mdzg_gcache* gc;            /* graphs of persistent batches */
+    /* tests: the launch plan numbered fail_plan (counting from 1, every
+       job's plan counts) fails as a device error would, after its copy-in
+       was queued (environment variable MDZG_FAILPOINT_PLAN read by
+       mdzg_create; 0 = never) */
+    long fail_plan;
+    long plans;
 };
 
 /* noalloc.hip: device address of the static scratch array of the current

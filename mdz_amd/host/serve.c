@@ -410,7 +410,7 @@ static int run_slice(mdzg_server* s)
                                 s->d_ctr + k, grid, s->slice_iters[k]) != 0)
             return MDZG_ERR_DEVICE;
     }
-    if (c->graphs) {
+    if (c->graphs && nk >= 2) {
         if (mdzg_graph_run(c->gc, &reset, par, nk, NULL, st) != 0)
             return MDZG_ERR_DEVICE;
     } else {

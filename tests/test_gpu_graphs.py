@@ -1,8 +1,8 @@
 """GPU tests of graph replay, the options and the autotuner.
 
-A persistent batch (the sentinel fill of an asynchronous job, the concurrent
-class launches, an image job's colouring pass) runs as one replay of a cached
-HIP graph by default.  Each
+A persistent batch of two or more arithmetic classes (the sentinel fill of
+an asynchronous job, the concurrent class launches) runs as one replay of a
+cached HIP graph by default; a one-class plan is launched directly.  Each
 test replays graphs several times, with eager (one-by-one) renders and host
 synchronisation between the replays, and checks every replay bit for bit
 against the maps of MDZ's own loops."""
@@ -49,8 +49,9 @@ def test_batch_graph_replays_equal_eager_and_golden(ctx, golden):
     ctx.option_set(M.OPT_GRAPHS, 1)
 
 
-def test_single_tile_and_image_graphs(ctx, golden):
-    """One-class batches and image jobs (fill, kernel, colouring pass)."""
+def test_single_tile_and_image_jobs_with_either_option(ctx, golden):
+    """One-class batches and image jobs (kernel, colouring pass), launched
+    directly under either setting of the option."""
     (m, e), = _deep(golden, ["deep_mpfr128_f0"])
     v = _view(m["view"])
     rng = np.random.default_rng(3)
@@ -164,3 +165,27 @@ def test_async_jobs_fill_the_sentinel_in_the_graph(ctx, golden):
             for (m, e), got in zip(cases, job.counts):
                 assert _eq(got, e), (rep, graphs, m["name"])
     ctx.option_set(M.OPT_GRAPHS, 1)
+
+
+def test_failed_launch_releases_everything(golden, monkeypatch):
+    """A job whose launch plan fails (forced from the host by the plan
+    failpoint, after its copy-in was queued) returns the device error and
+    leaves the context whole: the renders before and after it, and enough
+    renders to cycle every cached buffer, equal the maps."""
+    (m, e), = _deep(golden, ["deep_mpfr176_f3"])
+    v = _view(m["view"])
+    monkeypatch.setenv("MDZG_FAILPOINT_PLAN", "3")
+    with M.Context(0) as c2:
+        monkeypatch.delenv("MDZG_FAILPOINT_PLAN")
+        for i in range(1, 13):
+            if i == 3:
+                with pytest.raises(M.MdzgError) as err:
+                    c2.render(v)
+                assert err.value.code == -3
+                continue
+            got, _ = c2.render(v)
+            assert _eq(got, e), i
+        job = c2.submit([(v, None, None)], fill=7)
+        rc, _ = job.wait()
+        job.free()
+        assert rc == 0 and _eq(job.counts[0], e)
