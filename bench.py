@@ -330,6 +330,7 @@ def config4_serving(ctx, M, WL, np, fracs=(0.25, 0.5, 0.75, 0.9), n=300):
     batching against one launch per request."""
     tiles, _ = crop_tiles(WL, n)
     out = {}
+    serve_run(ctx, M, np, M.SERVE_BATCHED, tiles[:60], float("inf"))     # warm-up
     for name, mode in (("batched", M.SERVE_BATCHED), ("per_request", M.SERVE_PER_REQUEST)):
         unl = unloaded_latency(ctx, M, np, mode, tiles)
         sat = serve_run(ctx, M, np, mode, tiles, float("inf"))
@@ -472,12 +473,31 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only a plain device-to-device copy of MIB MiB (calibrates "
                          "FETCH_SIZE / WRITE_SIZE against known bytes)")
+    ap.add_argument("--rate-probe", action="store_true",
+                    help="only the issue rates of v_mad_u64_u32 and v_fma_f64 (the roofline "
+                         "peak of the RN kernels)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the escape-time tiles shard across GPUs with no "
               "exchange, and this build runs on one device only (use --gpus 1).",
               file=sys.stderr)
         sys.exit(2)
+
+    if a.rate_probe:
+        import mdz_amd as M
+        with M.Context(0) as ctx:
+            out = {}
+            for b in (1, 2, 4, 8):
+                mad = ctx.rate_probe(0, b)
+                fma = ctx.rate_probe(1, b)
+                # lanes per CU per clock at the 2.4 GHz peak clock
+                out[f"blocks_per_cu_{b}"] = dict(
+                    mad_ops_per_s=mad, fma_ops_per_s=fma, mad_over_fma=mad / fma,
+                    mad_lanes_per_cu_clk=mad / 256 / 2.4e9, fma_lanes_per_cu_clk=fma / 256 / 2.4e9)
+                log(f"blocks/CU {b}: v_mad_u64_u32 {mad / 1e12:.2f} T lane-ops/s, "
+                    f"v_fma_f64 {fma / 1e12:.2f} T, ratio {mad / fma:.3f}")
+        print(json.dumps(out))
+        return
 
     if a.copy_probe:
         import torch

@@ -298,6 +298,14 @@ int mdzg_autotune(mdzg_ctx* ctx, const mdzg_tile* probes, int nprobes, int reps,
 int mdzg_tune_save(mdzg_ctx* ctx, const char* path);
 int mdzg_tune_load(mdzg_ctx* ctx, const char* path);
 
+/* ---- issue-rate probe (no counterpart in the reference) ---------------
+   The roofline peak of the RN kernels: *ops_per_s of v_mad_u64_u32 (kind 0,
+   32x32+64 multiply-accumulates, the products' instruction) or v_fma_f64
+   (kind 1) lane operations, measured with blocks_per_cu resident 256-lane
+   blocks per CU, four independent chains per lane.  Allocates a small
+   buffer of its own. */
+int mdzg_rate_probe(mdzg_ctx* ctx, int kind, int blocks_per_cu, double* ops_per_s);
+
 /* ---- asynchronous form (submit / poll / wait / cancel) ---------------- */
 typedef struct mdzg_job mdzg_job;
 int  mdzg_submit(mdzg_ctx* ctx, const mdzg_tile* tiles, int ntiles, mdzg_job** out);

@@ -183,6 +183,7 @@ def lib():
         L.mdzg_autotune.argtypes = [vp, P(Tile), ctypes.c_int, ctypes.c_int, P(TuneResult)]
         L.mdzg_tune_save.argtypes = [vp, ctypes.c_char_p]
         L.mdzg_tune_load.argtypes = [vp, ctypes.c_char_p]
+        L.mdzg_rate_probe.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
         L.mdzg_kernel_id.argtypes = [ctypes.c_int, ctypes.c_long]
         L.mdzg_tune_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.mdzg_tune_get.argtypes = [vp, ctypes.c_int] + [P(ctypes.c_int)] * 3
@@ -435,6 +436,13 @@ class Context:
         r = TuneResult()
         _check(lib().mdzg_autotune(self._c, arr, len(views), reps, ctypes.byref(r)), "autotune")
         return r.as_dict()
+
+    def rate_probe(self, kind, blocks_per_cu=8):
+        """Lane operations per second of v_mad_u64_u32 (kind 0) or v_fma_f64 (1)."""
+        v = ctypes.c_double()
+        _check(lib().mdzg_rate_probe(self._c, kind, blocks_per_cu, ctypes.byref(v)),
+               "rate_probe")
+        return v.value
 
     def tune_save(self, path):
         _check(lib().mdzg_tune_save(self._c, _b(path)), "tune_save")

@@ -56,6 +56,42 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint64_t n, uint3
         p[i] = v;
 }
 
+/* Issue-rate probes for the roofline peak: every lane runs four independent
+   chains of one instruction, v_mad_u64_u32 (32x32+64, the multiply-accumulate
+   of the RN products) or v_fma_f64, `iters` times.  The chains never stop on
+   a data value; the result is stored so nothing is removed. */
+__global__ __launch_bounds__(256) void k_probe_mad(uint32_t* out, int iters)
+{
+    uint64_t a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3;
+    const uint32_t x = blockIdx.x | 1u, y = threadIdx.x | 3u;
+    for (int i = 0; i < iters; ++i) {
+        uint64_t c0, c1, c2, c3;
+        asm volatile("v_mad_u64_u32 %0, %4, %8, %9, %0\n\t"
+                     "v_mad_u64_u32 %1, %5, %8, %9, %1\n\t"
+                     "v_mad_u64_u32 %2, %6, %8, %9, %2\n\t"
+                     "v_mad_u64_u32 %3, %7, %8, %9, %3"
+                     : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "=&s"(c0), "=&s"(c1), "=&s"(c2),
+                       "=&s"(c3)
+                     : "v"(x), "v"(y));
+    }
+    out[blockIdx.x * blockDim.x + threadIdx.x] = (uint32_t)(a0 ^ a1 ^ a2 ^ a3);
+}
+
+__global__ __launch_bounds__(256) void k_probe_fma(uint32_t* out, int iters)
+{
+    double a0 = threadIdx.x, a1 = a0 + 1.0, a2 = a0 + 2.0, a3 = a0 + 3.0;
+    const double x = 0.999999, y = 1e-9 * (blockIdx.x + 1);
+    for (int i = 0; i < iters; ++i) {
+        asm volatile("v_fma_f64 %0, %4, %0, %5\n\t"
+                     "v_fma_f64 %1, %4, %1, %5\n\t"
+                     "v_fma_f64 %2, %4, %2, %5\n\t"
+                     "v_fma_f64 %3, %4, %3, %5"
+                     : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3)
+                     : "v"(x), "v"(y));
+    }
+    out[blockIdx.x * blockDim.x + threadIdx.x] = (uint32_t)(int)(a0 + a1 + a2 + a3);
+}
+
 static void node_init(mdzg_gnode* n, const void* f, unsigned grid, unsigned block)
 {
     memset(n, 0, sizeof *n);
@@ -77,6 +113,41 @@ static void node_arg(mdzg_gnode* n, const T& v)
 extern "C" {
 
 int mdzg_dev_nkernels(void) { return MDZG_NKERNELS; }
+
+/* kind 0: v_mad_u64_u32, 1: v_fma_f64.  blocks of 256 lanes (4 waves, one
+   per SIMD) per CU x ncu, iters x 4 instructions per lane; *ms the kernel
+   time by HIP events on a stream of its own. */
+int mdzg_dev_rate_probe(int kind, int ncu, int blocks_per_cu, int iters, double* ms)
+{
+    if ((kind != 0 && kind != 1) || ncu < 1 || blocks_per_cu < 1 || iters < 1 || !ms)
+        return -1;
+    const unsigned grid = (unsigned)ncu * (unsigned)blocks_per_cu;
+    uint32_t* out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = -3;
+    if (hipMalloc((void**)&out, (size_t)grid * 256 * 4) == hipSuccess &&
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+        hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        const void* f = kind == 0 ? (const void*)k_probe_mad : (const void*)k_probe_fma;
+        void* args[] = { (void*)&out, (void*)&iters };
+        float t = 0.f;
+        if (hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, st) == hipSuccess &&   /* warm-up */
+            hipEventRecord(e0, st) == hipSuccess &&
+            hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, st) == hipSuccess &&
+            hipEventRecord(e1, st) == hipSuccess && hipStreamSynchronize(st) == hipSuccess &&
+            hipEventElapsedTime(&t, e0, e1) == hipSuccess) {
+            *ms = t;
+            rc = 0;
+        }
+    }
+    if (st) hipStreamSynchronize(st);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    if (out) hipFree(out);
+    return rc;
+}
 
 int mdzg_dev_node_fill(mdzg_gnode* n, uint32_t* p, uint64_t count, uint32_t v)
 {

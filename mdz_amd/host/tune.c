@@ -221,3 +221,17 @@ int mdzg_autotune(mdzg_ctx* c, const mdzg_tile* probes, int nprobes, int reps,
     }
     return rc;
 }
+
+int mdzg_rate_probe(mdzg_ctx* c, int kind, int blocks_per_cu, double* ops_per_s)
+{
+    if (!c || !ops_per_s || (kind != 0 && kind != 1) || blocks_per_cu < 1 || blocks_per_cu > 32)
+        return MDZG_ERR_ARG;
+    hipSetDevice(c->device);
+    const int iters = 4096;
+    double ms = 0.0;
+    if (mdzg_dev_rate_probe(kind, c->ncu, blocks_per_cu, iters, &ms) != 0 || ms <= 0.0)
+        return MDZG_ERR_DEVICE;
+    const double lanes = (double)c->ncu * blocks_per_cu * 256.0;
+    *ops_per_s = lanes * 4.0 * iters / (ms * 1e-3);
+    return MDZG_OK;
+}
