@@ -223,6 +223,22 @@ def gmp_path(ctx, M, WL, a, cpu=True):
                          "640x480 depth 5000", kernel="k_gmp_persist<4>"),
              kernel_gpix_per_s=pis[0] / (sum(kms) / len(kms) * 1e-3) / 1e9)
     log(f"gmp path: {r['kernel_gpix_per_s']:.2f} G pixel-iter/s (kernel)")
+    # rows of the same view at the other storage buckets' precisions, the
+    # MPFR path on the same rows for scale (96 rows at p <= 256, 32 above)
+    bands = {}
+    for p in (256, 512, 1024):
+        rows = 96 if p <= 256 else 32
+        out = {}
+        for name, path in (("gmp", M.PATH_GMP), ("mpfr", M.PATH_MPFR)):
+            rp = M.rect_from_center(WL.CONFIG2_CENTRE[0], WL.CONFIG2_CENTRE[1], "1e-25", p, 640,
+                                    480, coords=(path == M.PATH_GMP))
+            vp = M.make_view(rp, precision=p, depth=5000, width=640, height=480, path=path)
+            ctx.render(vp, lines=(0, 4))
+            _, st = ctx.render(vp, lines=(0, rows))
+            out[name] = st["pixel_iters"] / (st["kernel_ms"] * 1e-3) / 1e9
+        bands[f"p{p}"] = dict(rows=rows, gmp_gpix_per_s=out["gmp"], mpfr_gpix_per_s=out["mpfr"])
+        log(f"gmp p={p}: {out['gmp']:.2f} G pixel-iter/s, mpfr path {out['mpfr']:.2f}")
+    r["bands"] = bands
     if cpu:
         r["cpu_baseline"] = cpu_baseline(list(rect), 128, 5000, 640, 480, 16, host_cores(),
                                          path=3)
@@ -473,6 +489,9 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only a plain device-to-device copy of MIB MiB (calibrates "
                          "FETCH_SIZE / WRITE_SIZE against known bytes)")
+    ap.add_argument("--op-probe", action="store_true",
+                    help="only the per-operation probe of the RN arithmetic at 128 and 1024 "
+                         "bits (for counter runs: one launch per operation, in OPP order)")
     ap.add_argument("--rate-probe", action="store_true",
                     help="only the issue rates of v_mad_u64_u32 and v_fma_f64 (the roofline "
                          "peak of the RN kernels)")
@@ -496,6 +515,31 @@ def main():
                     mad_lanes_per_cu_clk=mad / 256 / 2.4e9, fma_lanes_per_cu_clk=fma / 256 / 2.4e9)
                 log(f"blocks/CU {b}: v_mad_u64_u32 {mad / 1e12:.2f} T lane-ops/s, "
                     f"v_fma_f64 {fma / 1e12:.2f} T, ratio {mad / fma:.3f}")
+        print(json.dumps(out))
+        return
+
+    if a.op_probe:
+        import mdz_amd as M
+        from mdz_amd import workloads as WL
+        names = ("loop_only", "product", "square", "add_sub", "squares_diff", "escape",
+                 "iteration")
+        out = {}
+        with M.Context(0) as ctx:
+            for leg, view, lines, warm, reps in (
+                    ("rn128", WL.config2_view(), None, 1000, 4000),
+                    ("rn1024", WL.config5_view(), (1048, 16), 200, 200)):
+                H = lines[1] if lines else view.real_height
+                W = view.real_width
+                waves = (H * W + 63) // 64
+                r = dict(workload=f"{W}x{H} pixels of the "
+                                  f"{'config 2' if leg == 'rn128' else 'config 5'} view, "
+                                  f"{warm} iterations first", waves=waves, warm=warm, reps=reps,
+                         kernel_ms={})
+                for op, nm in enumerate(names):
+                    ms = ctx.op_probe(view, op, warm=warm, reps=reps, lines=lines)
+                    r["kernel_ms"][nm] = ms
+                    log(f"{leg} {nm}: {ms:.3f} ms")
+                out[leg] = r
         print(json.dumps(out))
         return
 
@@ -571,6 +615,23 @@ def main():
             ctx.close()
             print(json.dumps(res))
             return
+
+        # the same frame shape and arithmetic where pixels escape
+        ve = WL.config2_escaping_view()
+        ctx.render(ve)
+        ke, we = [], []
+        for _ in range(5):
+            _, st = ctx.render(ve)
+            ke.append(st["kernel_ms"])
+            we.append(st["wall_ms"])
+        kea = sum(ke) / len(ke)
+        res["rn128_escaping"] = dict(
+            workload="640x480 p=128 depth 5000, 1e-24 wide at 0.2926 + 0.4557i (37 % of pixels "
+                     "escape)", pixel_iters=st["pixel_iters"], kernel_ms=kea,
+            kernel_gpix_per_s=st["pixel_iters"] / (kea * 1e-3) / 1e9,
+            wall_gpix_per_s=st["pixel_iters"] / (float(np.median(we)) * 1e-3) / 1e9)
+        log(f"rn128 escaping view: {res['rn128_escaping']['kernel_gpix_per_s']:.1f} G "
+            f"pixel-iter/s (kernel)")
 
         # FP64 figure: config 1 frame
         v1 = WL.config1_view()

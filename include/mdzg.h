@@ -306,6 +306,22 @@ int mdzg_tune_load(mdzg_ctx* ctx, const char* path);
    buffer of its own. */
 int mdzg_rate_probe(mdzg_ctx* ctx, int kind, int blocks_per_cu, double* ops_per_s);
 
+/* ---- per-operation probe (no counterpart in the reference) -------------
+   Measurement only.  One lane per pixel of the tile (MPFR path, p = 128 or
+   1024: the 128- and 1024-bit legs); each lane runs `warm` iterations of
+   the escape loop from its pixel, then repeats one RN operation `reps`
+   times in a dependent chain (op: 0 loop only, 1 product, 2 square,
+   3 addition of c_re and its subtraction (two a rep), 4 difference of the
+   squares, 5 escape test, 6 whole iteration).  *kernel_ms is the launch's
+   time; instruction counters of the launch divided by its waves and reps
+   give the operation's cost.  tile->counts is not used (the probe writes a
+   buffer of its own).  MDZG_ERR_RANGE for another path or precision. */
+enum { MDZG_OPP_LOOP_ONLY = 0, MDZG_OPP_PRODUCT = 1, MDZG_OPP_SQUARE = 2,
+       MDZG_OPP_ADD_SUB = 3, MDZG_OPP_SQUARES_DIFF = 4, MDZG_OPP_ESCAPE = 5,
+       MDZG_OPP_ITERATION = 6 };
+int mdzg_op_probe(mdzg_ctx* ctx, const mdzg_tile* tile, int op, int warm, int reps,
+                  double* kernel_ms);
+
 /* ---- asynchronous form (submit / poll / wait / cancel) ---------------- */
 typedef struct mdzg_job mdzg_job;
 int  mdzg_submit(mdzg_ctx* ctx, const mdzg_tile* tiles, int ntiles, mdzg_job** out);

@@ -91,6 +91,9 @@ class TuneResult(ctypes.Structure):
 
 
 OPT_GRAPHS, OPT_BAND_PIXELS, OPT_BAND_INFLIGHT = 1, 2, 3
+# operations of Context.op_probe (mdzg_op_probe)
+OPP_LOOP_ONLY, OPP_PRODUCT, OPP_SQUARE, OPP_ADD_SUB, OPP_SQUARES_DIFF, OPP_ESCAPE, \
+    OPP_ITERATION = range(7)
 
 
 class ServerCfg(ctypes.Structure):
@@ -184,6 +187,8 @@ def lib():
         L.mdzg_tune_save.argtypes = [vp, ctypes.c_char_p]
         L.mdzg_tune_load.argtypes = [vp, ctypes.c_char_p]
         L.mdzg_rate_probe.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
+        L.mdzg_op_probe.argtypes = [vp, P(Tile), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    P(ctypes.c_double)]
         L.mdzg_kernel_id.argtypes = [ctypes.c_int, ctypes.c_long]
         L.mdzg_tune_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.mdzg_tune_get.argtypes = [vp, ctypes.c_int] + [P(ctypes.c_int)] * 3
@@ -442,6 +447,15 @@ class Context:
         v = ctypes.c_double()
         _check(lib().mdzg_rate_probe(self._c, kind, blocks_per_cu, ctypes.byref(v)),
                "rate_probe")
+        return v.value
+
+    def op_probe(self, view, op, warm=1000, reps=1000, lines=None, cols=None):
+        """mdzg_op_probe: kernel ms of one RN operation (OPP_*) repeated reps
+        times per lane after warm iterations of each pixel's orbit."""
+        t, _ = self._tile(view, lines, cols)
+        v = ctypes.c_double()
+        _check(lib().mdzg_op_probe(self._c, ctypes.byref(t), op, warm, reps, ctypes.byref(v)),
+               "op_probe")
         return v.value
 
     def tune_save(self, path):

@@ -616,6 +616,85 @@ __global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_tile(cons
     t.out[idx] = count;
 }
 
+/* Per-operation probe (measurement only, p = 32N).  Each lane takes its
+   pixel of the tile, runs `warm` iterations of the escape loop so that its
+   values carry the orbit's exponents and bit patterns (lanes of a wave
+   neighbours, as in a render), then repeats one operation `reps` times in a
+   dependent chain whose operands keep those statistics:
+     MDZG_OPP_LOOP   the loop alone (the chain's register dependency only)
+     MDZG_OPP_MUL    x = RN(x * y)
+     MDZG_OPP_SQR    x = RN(x^2) with the exponent set back to 0
+     MDZG_OPP_ADD    x = RN(x + c_re), then x = RN(x - c_re): two a rep
+     MDZG_OPP_SUBNN  a = |RN(a - b)|, a and b the squares (the SUB_NONNEG form)
+     MDZG_OPP_ESC    the escape test on the squares
+     MDZG_OPP_STEP   the whole iteration
+   Counter passes over the launches give each operation's instructions.  A
+   word of the result goes to the tile's output so that nothing is removed;
+   it is not a count. */
+template <int N>
+__global__ __launch_bounds__(256, (rn_min_waves<N, true>())) void k_rn_op_probe(
+    const mdzg_dtile* tp, int op, int warm, int reps)
+{
+    const mdzg_dtile& t = *tp;
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = (uint32_t)t.ncols * (uint32_t)t.nrows;
+    if (idx >= npix)
+        return;
+    const int row = (int)(idx / (uint32_t)t.ncols), col = (int)(idx % (uint32_t)t.ncols);
+    __shared__ uint32_t c_lds[c_in_lds<N>() ? 2 * (N + 2) * 256 : 1];
+    rn_state<N> s;
+    s.c.init(c_lds);
+    rn_init<N, true>(s, t, row, col);
+    const int fractal = t.fractal;
+    for (int32_t wz = 1; wz <= warm; ++wz)
+        (void)rn_step<N, true>(s, fractal, wz, 0);
+    const double thresh = 4.0 + ldexp(1.0, 2 - 32 * N);
+    uint32_t h = 0;
+    mp<N> x = s.wre, y = s.wim;
+    /* one loop per operation, so that no loop carries the other
+       operations' values */
+    switch (op) {
+    case MDZG_OPP_MUL:
+        for (int i = 0; i < reps; ++i)
+            mul_rn<N, false>(x, x, y, 0);
+        break;
+    case MDZG_OPP_SQR:
+        for (int i = 0; i < reps; ++i) {
+            mul_rn<N, true>(x, x, x, 0);
+            x.e = is_zero(x) ? x.e : 0;
+        }
+        break;
+    case MDZG_OPP_ADD:
+        for (int i = 0; i < reps; ++i) {
+            add_rn<N>(x, x, s.c.get_re(), 0u, 0);
+            add_rn<N>(x, x, s.c.get_re(), 1u, 0);
+        }
+        break;
+    case MDZG_OPP_SUBNN:
+        for (int i = 0; i < reps; ++i) {
+            add_rn<N, SUB_NONNEG>(x, s.wre2, s.wim2, 1u, 0);
+            x.s = 0;
+            s.wre2 = x;
+        }
+        break;
+    case MDZG_OPP_ESC:
+        for (int i = 0; i < reps; ++i) {
+            h += escaped<N>(s.wim2, s.wre2, thresh, 0) ? 1u : 0u;
+            asm volatile("" : "+v"(s.wim2.m[0]));
+        }
+        break;
+    case MDZG_OPP_STEP:
+        for (int i = 0; i < reps; ++i)
+            h += rn_step<N, true>(s, fractal, i, 0) ? 1u : 0u;
+        break;
+    default:
+        for (int i = 0; i < reps; ++i)
+            asm volatile("" : "+v"(x.m[0]));
+        break;
+    }
+    t.out[idx] = h ^ x.m[N - 1] ^ s.wre.m[N - 1] ^ s.wre2.m[N - 1];
+}
+
 template <int N, bool FULL>
 __global__ __launch_bounds__(256, (rn_min_waves<N, FULL>())) void k_rn_persist(const mdzg_dtile* tiles, int ntiles,
                                                     uint64_t total,

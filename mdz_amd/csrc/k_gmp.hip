@@ -72,15 +72,22 @@ __device__ __forceinline__ bool gmp_step(gmp_state<L>& s, int fractal, int32_t w
     if (fractal == F_BURNING_SHIP)
         gf_abs<L>(t1);
     gf_mul2<L>(t2, t1, lr);                                /* t1 *= 2 (mul_ui)   */
-    gf_add<L>(s.wim, t2, s.cim, 0u, lr);                   /* wim = t1 + c_im    */
-    gf_sub<L>(t1, s.wre2, s.wim2, lr);                     /* t1 = wre2 - wim2   */
+    gf_addsub<L>(s.wim, t2, s.cim, 0u, lr);                /* wim = t1 + c_im    */
+    gf_addsub<L>(t1, s.wre2, s.wim2, 1u, lr);              /* t1 = wre2 - wim2   */
     if (fractal == F_GENERALIZED_CELTIC || (fractal == F_VARIANT && (wz & 1)))
         gf_abs<L>(t1);
-    gf_add<L>(s.wre, t1, s.cre, 0u, lr);                   /* wre = t1 + c_re    */
+    gf_addsub<L>(s.wre, t1, s.cre, 0u, lr);                /* wre = t1 + c_re    */
     gf_sqr<L>(s.wim2, s.wim, lr);                          /* wim2 = wim^2       */
     gf_sqr<L>(s.wre2, s.wre, lr);                          /* wre2 = wre^2       */
-    gf_add<L>(t1, s.wim2, s.wre2, 0u, lr);                 /* mpf_cmp (.., 4)    */
-    return gf_gt4<L>(t1);
+    /* t1 = wim2 + wre2, mpf_cmp (t1, 4): the sum is formed only where the
+       integer limbs leave the comparison open */
+    const int q = gf_sum_gt4_quick<L>(s.wim2, s.wre2);
+    bool esc = q > 0;
+    if (gf_any(q < 0)) {
+        gf_add<L>(t1, s.wim2, s.wre2, 0u, lr);
+        esc = (q < 0) ? gf_gt4<L>(t1) : esc;
+    }
+    return esc;
 }
 
 template <int L>

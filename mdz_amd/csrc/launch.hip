@@ -24,6 +24,8 @@ void mdzg_kget_rn16(int, const void**, const void**, const void**);
 void mdzg_kget_rn24(int, const void**, const void**, const void**);
 void mdzg_kget_rn32(int, const void**, const void**, const void**);
 void mdzg_kget_gmp(int bucket, const void**, const void**, const void**);
+const void* mdzg_kprobe_rn4(void);
+const void* mdzg_kprobe_rn32(void);
 }
 
 typedef void (*kget_fn)(int, const void**, const void**, const void**);
@@ -216,6 +218,19 @@ int mdzg_dev_launch_tile(int kid, const mdzg_dtile* d_tile, uint32_t npix, hipSt
     const void* f;
     lookup(kid, 0, &f);
     void* args[] = { (void*)&d_tile };
+    return hipLaunchKernel(f, dim3((npix + 255u) / 256u), dim3(256), args, 0, st) == hipSuccess
+               ? 0 : -3;
+}
+
+/* the per-operation probe of the RN kernel of nwords (4 or 32) words, one
+   lane per pixel of the tile */
+int mdzg_dev_launch_op_probe(int nwords, const mdzg_dtile* d_tile, uint32_t npix, int op,
+                             int warm, int reps, hipStream_t st)
+{
+    const void* f = nwords == 4 ? mdzg_kprobe_rn4() : (nwords == 32 ? mdzg_kprobe_rn32() : nullptr);
+    if (!f || npix == 0 || op < 0 || op >= MDZG_NOPP)
+        return -1;
+    void* args[] = { (void*)&d_tile, (void*)&op, (void*)&warm, (void*)&reps };
     return hipLaunchKernel(f, dim3((npix + 255u) / 256u), dim3(256), args, 0, st) == hipSuccess
                ? 0 : -3;
 }

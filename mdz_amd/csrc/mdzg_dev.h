@@ -29,6 +29,10 @@
    GMP bucket j */
 #define MDZG_NKERNELS (1 + 2 * MDZG_NCLASS_N + MDZG_NGMP)
 #define MDZG_KID_GMP0 (1 + 2 * MDZG_NCLASS_N)
+
+/* operations of the RN per-operation probe (k_rn_op_probe) */
+enum { MDZG_OPP_LOOP = 0, MDZG_OPP_MUL = 1, MDZG_OPP_SQR = 2, MDZG_OPP_ADD = 3,
+       MDZG_OPP_SUBNN = 4, MDZG_OPP_ESC = 5, MDZG_OPP_STEP = 6, MDZG_NOPP = 7 };
 /* GMP-path stored value of L limbs: 2L words (limbs from the most
    significant, low word first), size, exponent (limbs), sign; padded to 4 */
 #define MDZG_GF_VS(L) (((2 * (L) + 3) + 3) & ~3)

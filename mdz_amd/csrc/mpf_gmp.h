@@ -170,6 +170,30 @@ GF_HD limb pick(const limb (&a)[L], int i)
     return x;
 }
 
+/* columns Q .. 2W-2 of the product of the W-word numbers UW and VW (least
+   significant word first) into ZW, each column's products added in
+   multiply-accumulate groups (mp_rn.h mac_col) to the 96-bit accumulator
+   that carries the column below */
+template <int W, int Q>
+GF_HD void word_cols(const uint32_t* UW, const uint32_t* VW, uint32_t* ZW, uint64_t& acc)
+{
+    if constexpr (Q < 2 * W - 1) {
+        constexpr int i0 = Q < W ? 0 : Q - W + 1;
+        constexpr int n = Q < W ? Q + 1 : 2 * W - 1 - Q;
+        uint32_t ca[n], cb[n];
+#pragma unroll
+        for (int t = 0; t < n; ++t) {
+            ca[t] = UW[i0 + t];
+            cb[t] = VW[Q - i0 - t];
+        }
+        uint32_t c2;
+        mdzg::mac_col<n, true>(ca, cb, acc, c2);
+        ZW[Q] = (uint32_t)acc;
+        acc = (acc >> 32) | ((uint64_t)c2 << 32);
+        word_cols<W, Q + 1>(UW, VW, ZW, acc);
+    }
+}
+
 /* Z (2L limbs, most significant first) = U * V, where U and V (L limbs,
    most significant first) are zero below their top K limbs: the 2K-word
    numbers of those limbs (32-bit words, least significant first) multiplied
@@ -188,18 +212,7 @@ GF_HD void word_product(const limb (&U)[L], const limb (&V)[L], limb (&Z)[2 * L]
     }
     uint32_t ZW[2 * W];
     uint64_t acc = 0;
-    uint32_t c2 = 0;
-#pragma unroll
-    for (int q = 0; q < 2 * W - 1; ++q) {
-        const int i0 = q < W ? 0 : q - W + 1;
-        const int i1 = q < W ? q : W - 1;
-        mdzg::mac96_first(UW[i0], VW[q - i0], acc, c2);
-#pragma unroll
-        for (int i = i0 + 1; i <= i1; ++i)
-            mdzg::mac96(UW[i], VW[q - i], acc, c2);
-        ZW[q] = (uint32_t)acc;
-        acc = (acc >> 32) | ((uint64_t)c2 << 32);
-    }
+    word_cols<W, 0>(UW, VW, ZW, acc);
     ZW[2 * W - 1] = (uint32_t)acc;
 #pragma unroll
     for (int m = 0; m < 2 * L; ++m) {
@@ -621,6 +634,123 @@ GF_HD void gf_sub(gf<L>& r, const gf<L>& a, const gf<L>& b, int lr)
         return;
     }
     gf_sub_mag<L>(r, a, b, a.neg, lr);
+}
+
+/* ------------------------------------------------------------------ */
+/* r = a + (-1)^flipb b: mpf_add and mpf_sub in one carry chain.         */
+/*                                                                        */
+/* The common case of both -- nonzero operands and, for a subtraction of  */
+/* magnitudes, neither equal leading limbs at equal exponents nor one of  */
+/* mpf_sub's two near-cancellation forms -- has one shape: U (the operand */
+/* of larger exponent; of larger leading limb at equal exponents when     */
+/* subtracting) cut to `lim` limbs (PREC for an addition, PREC + 1 for a  */
+/* subtraction), V cut to the same window, for a subtraction both         */
+/* stripped of low zero limbs, V shifted down by the exponent gap, then   */
+/* U + V, or U + ~V + 1 = U - V.  An addition's carry out adds a leading  */
+/* 1 limb; a subtraction loses at most one leading limb (U's top limb is  */
+/* at least one unit above V's contribution outside the excluded forms).  */
+/* Lanes outside the common case take gf_add's general code, which a     */
+/* wave runs only when one of its lanes needs it.                         */
+/* ------------------------------------------------------------------ */
+template <int L>
+GF_HD void gf_addsub(gf<L>& r, const gf<L>& a, const gf<L>& b, uint32_t flipb, int lr)
+{
+    const uint32_t bneg = b.neg ^ flipb;
+    const bool sub = a.neg != bneg;
+    const bool sw = (a.exp < b.exp) | (sub & (a.exp == b.exp) & (a.t[0] < b.t[0]));
+    /* values first: selects between the operands' fields through their
+       addresses would keep the operands out of registers */
+    limb A[L], B[L], U[L], V[L];
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        A[i] = a.t[i];
+        B[i] = b.t[i];
+        U[i] = A[i];
+        V[i] = B[i];
+    }
+    if (gf_any(sw)) {
+#pragma unroll
+        for (int i = 0; i < L; ++i) {
+            U[i] = sw ? B[i] : A[i];
+            V[i] = sw ? A[i] : B[i];
+        }
+    }
+    const int uexp = sw ? b.exp : a.exp;
+    const int ediff = sw ? b.exp - a.exp : a.exp - b.exp;
+    int us = sw ? b.size : a.size, vs = sw ? a.size : b.size;
+    const bool near0 = (ediff == 0) & ((U[0] == V[0]) | (U[0] == V[0] + 1));
+    const bool near1 = (ediff == 1) & (U[0] == 1) & (V[0] == LMAX) & !((us >= 2) & (U[1] != 0));
+    const bool fast = (a.size != 0) & (b.size != 0) & !(sub & (near0 | near1));
+
+    const int lim = sub ? lr : lr - 1;
+    if (us > lim) us = lim;
+    if (vs + ediff > lim) vs = lim - ediff;      /* <= 0: V drops out */
+    keep_top<L>(U, us);
+    keep_top<L>(V, vs);
+    int ulast = 0, vlast = 0;                    /* the windows without low zero limbs */
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        ulast = (U[i] != 0) ? i + 1 : ulast;
+        vlast = (V[i] != 0) ? i + 1 : vlast;
+    }
+    const int ue = sub ? ulast : us;
+    const int ve = sub ? vlast : vs;
+    int rsize = (ve <= 0) ? us : ((ue > ediff) ? ((ve + ediff <= ue) ? ue : ve + ediff) : ve + ediff);
+    shift_down<L>(V, ediff);
+    limb T[L];
+    const limb m = sub ? LMAX : (limb)0;
+    limb c = sub ? 1 : 0;
+#pragma unroll
+    for (int k = L - 1; k >= 0; --k) T[k] = addc(U[k], V[k] ^ m, c, c);
+    int exp = uexp;
+    const bool grow = !sub & (c != 0);
+    if (gf_any(grow)) {
+#pragma unroll
+        for (int k = L - 1; k >= 1; --k) T[k] = grow ? T[k - 1] : T[k];
+        T[0] = grow ? (limb)1 : T[0];
+        rsize += grow ? 1 : 0;
+        exp += grow ? 1 : 0;
+    }
+    const bool nrm = sub & (T[0] == 0);
+    if (gf_any(nrm)) {
+#pragma unroll
+        for (int k = 0; k < L - 1; ++k) T[k] = nrm ? T[k + 1] : T[k];
+        T[L - 1] = nrm ? (limb)0 : T[L - 1];
+        rsize -= nrm ? 1 : 0;
+        exp -= nrm ? 1 : 0;
+    }
+    uint32_t neg = a.neg ^ ((sub & sw) ? 1u : 0u);
+    if (gf_any(!fast)) {
+        gf<L> s;
+        gf_add<L>(s, a, b, flipb, lr);
+#pragma unroll
+        for (int k = 0; k < L; ++k) T[k] = fast ? T[k] : s.t[k];
+        rsize = fast ? rsize : s.size;
+        exp = fast ? exp : s.exp;
+        neg = fast ? neg : s.neg;
+    }
+    /* r may be a or b: written last */
+#pragma unroll
+    for (int k = 0; k < L; ++k) r.t[k] = T[k];
+    r.size = rsize;
+    r.exp = exp;
+    r.neg = neg;
+}
+
+/* mpf_add (t, a, b); mpf_cmp (t, 4) > 0 for nonnegative a and b (the
+   squares): 1 or 0 when the integer limbs decide it, -1 otherwise.  The sum
+   of the operands' windows lies in [ia + ib, a + b] with ia, ib the integer
+   parts (the top limbs at exponent 1, which every window keeps), and
+   a + b < ia + ib + 2, so ia + ib <= 2 means no escape and ia + ib >= 5
+   (or an operand of 2^64 or more) an escape. */
+template <int L>
+GF_HD int gf_sum_gt4_quick(const gf<L>& a, const gf<L>& b)
+{
+    const bool big = ((a.size != 0) & (a.exp >= 2)) | ((b.size != 0) & (b.exp >= 2));
+    const limb ia = ((a.size != 0) & (a.exp == 1)) ? a.t[0] : (limb)0;
+    const limb ib = ((b.size != 0) & (b.exp == 1)) ? b.t[0] : (limb)0;
+    const bool high = big | (ia >= 5) | (ib >= 5) | (ia + ib >= 5);
+    return high ? 1 : ((ia + ib <= 2) ? 0 : -1);
 }
 
 }  // namespace gmpf

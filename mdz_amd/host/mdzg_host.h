@@ -24,6 +24,8 @@ int mdzg_dev_launch_persist(int kid, const mdzg_dtile* d_tiles, int ntiles, uint
                             int chunk, hipStream_t st);
 int mdzg_dev_occupancy_slice(int kid);
 int mdzg_dev_rate_probe(int kind, int ncu, int blocks_per_cu, int iters, double* ms);
+int mdzg_dev_launch_op_probe(int nwords, const mdzg_dtile* d_tile, uint32_t npix, int op,
+                             int warm, int reps, hipStream_t st);
 /* launch nodes (launch.hip, k_colour.hip) and their graph replay (graph.hip) */
 int mdzg_dev_node_fill(mdzg_gnode* n, uint32_t* p, uint64_t count, uint32_t v);
 int mdzg_dev_node_persist(mdzg_gnode* n, int kid, const mdzg_dtile* d_tiles, int ntiles,

@@ -935,6 +935,48 @@ int mdzg_render_tile(mdzg_ctx* c, const mdzg_tile* tile, mdzg_stats* st)
 static pthread_mutex_t na_lock[NA_MAX_DEVICES] = { [0 ... NA_MAX_DEVICES - 1] =
                                                        PTHREAD_MUTEX_INITIALIZER };
 
+/* bytes of a one-tile staging area: descriptor, queue counter and cancel
+   flag, then the start values */
+static size_t stage_bytes(const mdzg_tile* t, const mdzg_pview* pv)
+{
+    const size_t vsb = 4 * (size_t)pv->vs;
+    return NA_OFF_COORDS +
+           (pv->vs ? ALIGN_UP(((size_t)t->ncols + (size_t)t->nlines + 2) * vsb, 16) : 0);
+}
+
+/* the tile's descriptor and start values into the host image H of a
+   staging area whose device copy is D; counts to `out` (device memory) */
+static int stage_tile(const mdzg_tile* t, mdzg_pview* pv, char* H, char* D, uint32_t* out)
+{
+    const size_t vsb = 4 * (size_t)pv->vs;
+    memset(H, 0, NA_OFF_COORDS);          /* descriptor, counter and cancel flag */
+    mdzg_dtile* d = (mdzg_dtile*)H;
+    d->out = out;
+    d->ncols = t->ncols;
+    d->nrows = t->nlines;
+    d->col0 = t->col0;
+    d->line0 = t->line0;
+    d->W = t->view.real_width;
+    d->fractal = t->view.fractal;
+    d->family = t->view.family;
+    d->kclr = pv->kclr;
+    d->depth = (int32_t)t->view.depth;
+    d->vs = pv->vs;
+    d->gprec = pv->v.path == MDZG_PATH_GMP ? pv->nw : 0;
+    d->pix_begin = 0;
+    if (!pv->vs) {
+        mdzg_coords_fp64(pv, &d->xmin, &d->ymax, &d->width, &d->jre, &d->jim);
+        return MDZG_OK;
+    }
+    uint32_t* xs = (uint32_t*)(H + NA_OFF_COORDS);
+    uint32_t* ys = (uint32_t*)(H + NA_OFF_COORDS + vsb * (size_t)t->ncols);
+    uint32_t* jc = (uint32_t*)(H + NA_OFF_COORDS + vsb * ((size_t)t->ncols + t->nlines));
+    d->xs = (const uint32_t*)(D + NA_OFF_COORDS);
+    d->ys = (const uint32_t*)(D + NA_OFF_COORDS + vsb * (size_t)t->ncols);
+    d->jc = (const uint32_t*)(D + NA_OFF_COORDS + vsb * ((size_t)t->ncols + t->nlines));
+    return mdzg_coords_fill(pv, t->line0, t->nlines, t->col0, t->ncols, xs, ys, jc);
+}
+
 int mdzg_render_tile_noalloc(mdzg_ctx* c, const mdzg_tile* t, mdzg_stats* st)
 {
     const double t0 = mdzg_now_ms();
@@ -955,9 +997,7 @@ int mdzg_render_tile_noalloc(mdzg_ctx* c, const mdzg_tile* t, mdzg_stats* st)
     int rc = mdzg_view_parse(&t->view, &pv);
     if (rc != MDZG_OK)
         return rc;
-    const size_t vsb = 4 * (size_t)pv.vs;
-    const size_t need = NA_OFF_COORDS +
-        (pv.vs ? ALIGN_UP(((size_t)t->ncols + (size_t)t->nlines + 2) * vsb, 16) : 0);
+    const size_t need = stage_bytes(t, &pv);
     if (need > MDZG_NOALLOC_SCRATCH_BYTES) {
         mdzg_view_clear(&pv);
         return MDZG_ERR_RANGE;
@@ -965,32 +1005,7 @@ int mdzg_render_tile_noalloc(mdzg_ctx* c, const mdzg_tile* t, mdzg_stats* st)
     pthread_mutex_lock(&na_lock[c->device]);
     char* H = (char*)c->h_scratch;
     char* D = (char*)c->d_scratch;
-    memset(H, 0, NA_OFF_COORDS);          /* descriptor, counter and cancel flag */
-    mdzg_dtile* d = (mdzg_dtile*)H;
-    d->out = t->counts;
-    d->ncols = t->ncols;
-    d->nrows = t->nlines;
-    d->col0 = t->col0;
-    d->line0 = t->line0;
-    d->W = t->view.real_width;
-    d->fractal = t->view.fractal;
-    d->family = t->view.family;
-    d->kclr = pv.kclr;
-    d->depth = (int32_t)t->view.depth;
-    d->vs = pv.vs;
-    d->gprec = pv.v.path == MDZG_PATH_GMP ? pv.nw : 0;
-    d->pix_begin = 0;
-    if (pv.vs) {
-        uint32_t* xs = (uint32_t*)(H + NA_OFF_COORDS);
-        uint32_t* ys = (uint32_t*)(H + NA_OFF_COORDS + vsb * (size_t)t->ncols);
-        uint32_t* jc = (uint32_t*)(H + NA_OFF_COORDS + vsb * ((size_t)t->ncols + t->nlines));
-        rc = mdzg_coords_fill(&pv, t->line0, t->nlines, t->col0, t->ncols, xs, ys, jc);
-        d->xs = (const uint32_t*)(D + NA_OFF_COORDS);
-        d->ys = (const uint32_t*)(D + NA_OFF_COORDS + vsb * (size_t)t->ncols);
-        d->jc = (const uint32_t*)(D + NA_OFF_COORDS + vsb * ((size_t)t->ncols + t->nlines));
-    } else {
-        mdzg_coords_fp64(&pv, &d->xmin, &d->ymax, &d->width, &d->jre, &d->jim);
-    }
+    rc = stage_tile(t, &pv, H, D, t->counts);
     const int kid = pv.kid;
     mdzg_view_clear(&pv);
     if (rc == MDZG_OK) {
@@ -1018,5 +1033,59 @@ int mdzg_render_tile_noalloc(mdzg_ctx* c, const mdzg_tile* t, mdzg_stats* st)
         }
     }
     pthread_mutex_unlock(&na_lock[c->device]);
+    return rc;
+}
+
+/* The per-operation probe of the RN arithmetic (measurement only): the
+   tile's view must be on the MPFR path at p = 128 or 1024 (the 128- and
+   1024-bit legs, whole words); its pixels give the lanes their orbits. */
+_Static_assert(MDZG_OPP_PRODUCT == MDZG_OPP_MUL && MDZG_OPP_SQUARE == MDZG_OPP_SQR &&
+               MDZG_OPP_ADD_SUB == MDZG_OPP_ADD && MDZG_OPP_SQUARES_DIFF == MDZG_OPP_SUBNN &&
+               MDZG_OPP_ESCAPE == MDZG_OPP_ESC && MDZG_OPP_ITERATION == MDZG_OPP_STEP &&
+               MDZG_OPP_LOOP_ONLY == MDZG_OPP_LOOP, "probe operation numbers");
+
+int mdzg_op_probe(mdzg_ctx* c, const mdzg_tile* t, int op, int warm, int reps, double* kernel_ms)
+{
+    if (!c || !t || !kernel_ms || op < 0 || op >= MDZG_NOPP || warm < 0 || warm > 1000000 ||
+        reps < 1 || reps > 1000000 || t->ncols < 1 || t->nlines < 1 ||
+        (long long)t->ncols * t->nlines > (1ll << 24))
+        return MDZG_ERR_ARG;
+    if (t->view.path != MDZG_PATH_MPFR || (t->view.precision != 128 && t->view.precision != 1024))
+        return MDZG_ERR_RANGE;
+    hipSetDevice(c->device);
+    mdzg_pview pv;
+    int rc = mdzg_view_parse(&t->view, &pv);
+    if (rc != MDZG_OK)
+        return rc;
+    const uint32_t npix = (uint32_t)t->ncols * (uint32_t)t->nlines;
+    const size_t sb = ALIGN_UP(stage_bytes(t, &pv), 256);
+    char* H = (char*)malloc(sb);
+    char* D = NULL;
+    hipEvent_t e0 = NULL, e1 = NULL;
+    if (!H || hipMalloc((void**)&D, sb + 4 * (size_t)npix) != hipSuccess) {
+        free(H);
+        mdzg_view_clear(&pv);
+        return MDZG_ERR_NOMEM;
+    }
+    rc = stage_tile(t, &pv, H, D, (uint32_t*)(D + sb));
+    const int nwords = (int)t->view.precision / 32;
+    mdzg_view_clear(&pv);
+    float ms = 0.f;
+    if (rc == MDZG_OK &&
+        (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+         hipMemcpyAsync(D, H, sb, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+         hipEventRecord(e0, c->stream) != hipSuccess ||
+         mdzg_dev_launch_op_probe(nwords, (const mdzg_dtile*)D, npix, op, warm, reps,
+                                  c->stream) != 0 ||
+         hipEventRecord(e1, c->stream) != hipSuccess ||
+         hipStreamSynchronize(c->stream) != hipSuccess ||
+         hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = MDZG_ERR_DEVICE;
+    hipStreamSynchronize(c->stream);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(D);
+    free(H);
+    *kernel_ms = ms;
     return rc;
 }

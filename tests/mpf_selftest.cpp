@@ -1,8 +1,10 @@
 // Host build of mdz_amd/csrc/mpf_gmp.h checked against libgmp's mpf layer.
 //
 // Every operation of the GMP escape loop (src/frac_mandel.c:55-80 and
-// siblings) -- mpf_add, mpf_sub, mpf_mul (distinct operands and squares),
-// mpf_mul_ui by 2, mpf_abs, mpf_cmp with 4 -- is compared limb for limb
+// siblings) -- mpf_add and mpf_sub (each alone and both through the one
+// carry chain of gf_addsub), mpf_mul (distinct operands and squares),
+// mpf_mul_ui by 2, mpf_abs, mpf_cmp with 4 and the escape test's shortcut on
+// the integer limbs -- is compared limb for limb
 // (size, exponent, sign, limbs) with GMP on operands written straight into
 // mpf_t structures: random limbs, all-ones and zero limbs, equal leading
 // limbs, the x+1/x and 1 00../0 ff.. near-cancellation patterns, every
@@ -158,6 +160,38 @@ static void check(long cases)
         if (!same(r, ref)) report("add", P, a, b, r, ref);
         gf_sub<L>(r, a, b, LR); mpf_sub(R, A, B); from_gmp(ref, R);
         if (!same(r, ref)) report("sub", P, a, b, r, ref);
+        gf_addsub<L>(r, a, b, 1u, LR);
+        if (!same(r, ref)) report("addsub-sub", P, a, b, r, ref);
+        mpf_add(R, A, B); from_gmp(ref, R);
+        gf_addsub<L>(r, a, b, 0u, LR);
+        if (!same(r, ref)) report("addsub-add", P, a, b, r, ref);
+        {
+            // the escape test's shortcut on nonnegative operands
+            gf<L> x = a, y = b;
+            x.neg = 0;
+            y.neg = 0;
+            if (rnd64() % 2) {        // integer parts around the undecided 3..4
+                x.exp = 1; x.t[0] = rnd64() % 6 + (x.t[0] == 0);
+                if (x.size == 0) { x.size = 1; }
+                y.exp = (rnd64() % 3) ? 1 : (int)(rnd64() % 3) - 1;
+                if (y.size == 0) { y.size = 1; }
+                y.t[0] = (y.exp == 1) ? rnd64() % 6 + 1 : (y.t[0] | 1);
+            }
+            mpf_t X, Y, F;
+            mpf_init2(X, bits);
+            mpf_init2(Y, bits);
+            mpf_init2(F, bits);
+            mpf_set_si(F, 4);
+            to_gmp(X, x);
+            to_gmp(Y, y);
+            mpf_add(R, X, Y);
+            const int want = mpf_cmp(R, F) > 0;
+            const int q = gf_sum_gt4_quick<L>(x, y);
+            if (q >= 0 && q != want) report("sum-gt4-quick", P, x, y, x, y);
+            mpf_clear(X);
+            mpf_clear(Y);
+            mpf_clear(F);
+        }
         gf_mul<L>(r, a, b, LR); mpf_mul(R, A, B); from_gmp(ref, R);
         if (!same(r, ref)) report("mul", P, a, b, r, ref);
         gf_mul<L>(r, a, a, LR); mpf_mul(R, A, A); from_gmp(ref, R);
@@ -227,14 +261,17 @@ static void check_chain(long cases)
             if (fractal == 1) gf_abs<L>(t);
             gf<L> t2;
             gf_mul2<L>(t2, t, L);
-            gf_add<L>(zi, t2, ci, 0u, L);
-            gf_sub<L>(t, zr2, zi2, L);
+            gf_addsub<L>(zi, t2, ci, 0u, L);
+            gf_addsub<L>(t, zr2, zi2, 1u, L);
             if (fractal == 2 || (fractal == 3 && (wz % 2))) gf_abs<L>(t);
-            gf_add<L>(zr, t, cr, 0u, L);
+            gf_addsub<L>(zr, t, cr, 0u, L);
             gf_sqr<L>(zi2, zi, L);
             gf_sqr<L>(zr2, zr, L);
-            gf_add<L>(s, zi2, zr2, 0u, L);
-            const int esc = gf_gt4<L>(s) ? 1 : 0;
+            int esc = gf_sum_gt4_quick<L>(zi2, zr2);
+            if (esc < 0) {
+                gf_add<L>(s, zi2, zr2, 0u, L);
+                esc = gf_gt4<L>(s) ? 1 : 0;
+            }
 
             gf<L> rr, ri;
             from_gmp(rr, wre);
