@@ -269,10 +269,64 @@ __device__ __forceinline__ void fp64_run(fp64_state& s, int32_t& wz, uint64_t& l
         escm = start & ~live;
 }
 
-template <bool DEPTH>
+/* The latency form of an iteration, for launches whose pixels all fit the
+   device's lanes: each wave then runs alone on its SIMD, and the time of an
+   iteration is the length of its dependent chain.  The lane's live flag (lv)
+   and escape flag (ev) stay in vector registers, so that no scalar
+   instruction stands between one iteration's compare and the next
+   iteration's arithmetic (in the mask form, the count add of the next
+   iteration waits for the scalar update of the mask, which waits for the
+   compare); one more VALU instruction per iteration. */
+template <int F, bool DEPTH>
+__device__ __forceinline__ void fp64_iter_v(fp64_state& s, int32_t& wz, int32_t& lv, int32_t& ev,
+                                            int32_t depth)
+{
+    const bool esc = fp64_step(s, F, wz + 1);
+    wz += lv;
+    if (DEPTH) {
+        ev = esc ? (ev | lv) : ev;
+        lv = (esc || wz >= depth) ? 0 : lv;
+    } else {
+        lv = esc ? 0 : lv;
+    }
+}
+
+/* fp64_run in the latency form: the wave tests for an exit every 4
+   iterations, as fp64_run does */
+template <int F, bool DEPTH>
+__device__ __forceinline__ void fp64_run_v(fp64_state& s, int32_t& wz, uint64_t& live,
+                                           uint64_t& escm, int32_t depth, int n)
+{
+    const uint64_t start = live;
+    int32_t lv = (int32_t)((live >> __lane_id()) & 1u), ev = 0;
+    int it = 0;
+    for (; it + 4 <= n && __builtin_amdgcn_ballot_w64(lv != 0) != 0; it += 4) {
+        fp64_iter_v<F, DEPTH>(s, wz, lv, ev, depth);
+        fp64_iter_v<F, DEPTH>(s, wz, lv, ev, depth);
+        fp64_iter_v<F, DEPTH>(s, wz, lv, ev, depth);
+        fp64_iter_v<F, DEPTH>(s, wz, lv, ev, depth);
+    }
+    for (; it < n && __builtin_amdgcn_ballot_w64(lv != 0) != 0; ++it)
+        fp64_iter_v<F, DEPTH>(s, wz, lv, ev, depth);
+    live = __builtin_amdgcn_ballot_w64(lv != 0);
+    escm = DEPTH ? (escm | __builtin_amdgcn_ballot_w64(ev != 0)) : (start & ~live);
+}
+
+template <bool DEPTH, bool LAT>
 __device__ __forceinline__ void fp64_run_formula(int f, fp64_state& s, int32_t& wz, uint64_t& live,
                                                  uint64_t& escm, int32_t depth, int n)
 {
+    if (LAT) {
+        switch (f) {
+        case F_MANDELBROT: fp64_run_v<F_MANDELBROT, DEPTH>(s, wz, live, escm, depth, n); break;
+        case F_BURNING_SHIP: fp64_run_v<F_BURNING_SHIP, DEPTH>(s, wz, live, escm, depth, n); break;
+        case F_GENERALIZED_CELTIC:
+            fp64_run_v<F_GENERALIZED_CELTIC, DEPTH>(s, wz, live, escm, depth, n);
+            break;
+        default: fp64_run_v<F_VARIANT, DEPTH>(s, wz, live, escm, depth, n); break;
+        }
+        return;
+    }
     switch (f) {
     case F_MANDELBROT: fp64_run<F_MANDELBROT, DEPTH>(s, wz, live, escm, depth, n); break;
     case F_BURNING_SHIP: fp64_run<F_BURNING_SHIP, DEPTH>(s, wz, live, escm, depth, n); break;
@@ -280,6 +334,24 @@ __device__ __forceinline__ void fp64_run_formula(int f, fp64_state& s, int32_t& 
         fp64_run<F_GENERALIZED_CELTIC, DEPTH>(s, wz, live, escm, depth, n);
         break;
     default: fp64_run<F_VARIANT, DEPTH>(s, wz, live, escm, depth, n); break;
+    }
+}
+
+/* One run of `chunk` iterations for the wave.  No live lane passes its
+   depth in the first rem iterations (rem >= 1: an active lane has
+   wz < depth): those run without the depth test; lanes that reached their
+   depth there leave the mask (count 0 unless they escaped, which already
+   took them out), and the rest of the run has the test. */
+template <bool LAT>
+__device__ __forceinline__ void fp64_chunk(int f0, fp64_state& s, int32_t& wz, uint64_t& live,
+                                           uint64_t& escm, int32_t depth, int chunk, int rem)
+{
+    if (rem >= chunk) {
+        fp64_run_formula<false, LAT>(f0, s, wz, live, escm, depth, chunk);
+    } else {
+        fp64_run_formula<false, LAT>(f0, s, wz, live, escm, depth, rem);
+        live &= ~__builtin_amdgcn_ballot_w64(wz >= depth);
+        fp64_run_formula<true, LAT>(f0, s, wz, live, escm, depth, chunk - rem);
     }
 }
 
@@ -354,18 +426,10 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
             /* add_lanes' asm reads live: a scalar copy of the ballot (later
                iterations update it with scalar instructions) */
             uint64_t live = scalar_copy(am), escm = 0;
-            if (rem >= chunk) {
-                fp64_run_formula<false>(f0, s, wz, live, escm, depth, chunk);
-            } else {
-                /* no live lane passes its depth in the first rem iterations
-                   (rem >= 1: an active lane has wz < depth): those run without
-                   the depth test; lanes that reached their depth there leave
-                   the mask (count 0 unless they escaped, which already took
-                   them out), and the rest of the run has the test */
-                fp64_run_formula<false>(f0, s, wz, live, escm, depth, rem);
-                live &= ~__builtin_amdgcn_ballot_w64(wz >= depth);
-                fp64_run_formula<true>(f0, s, wz, live, escm, depth, chunk - rem);
-            }
+            if (by_lane)
+                fp64_chunk<true>(f0, s, wz, live, escm, depth, chunk, rem);
+            else
+                fp64_chunk<false>(f0, s, wz, live, escm, depth, chunk, rem);
             const bool esc = (escm >> __lane_id()) & 1u;
             if (active && (esc || wz >= depth)) {
                 put_count(out, esc ? (uint32_t)wz : 0u);

@@ -548,8 +548,11 @@ MP_HD void mul_cols(const uint32_t* a, const uint32_t* b, uint32_t* H, uint64_t&
             ca[t] = a[i0 + t];
             cb[t] = b[K - i0 - t];
         }
-        uint32_t c2;
-        mac_col<n, true>(ca, cb, acc, c2);
+        uint32_t c2 = 0;
+        if constexpr (K == 2 * N - 2)
+            acc += (uint64_t)ca[0] * cb[0];   /* top column: the sum fits 64 bits */
+        else
+            mac_col<n, true>(ca, cb, acc, c2);
         if (K < N - 1)
             st |= (uint32_t)acc;
         else
@@ -601,7 +604,9 @@ MP_HD void sqr_cols(const uint32_t* a, uint32_t* H, uint64_t& acc, uint32_t& st)
             c2 = d2 + cc;
             acc = ((uint64_t)l1 << 32) | l0;
         }
-        if constexpr ((K & 1) == 0)
+        if constexpr (K == 2 * N - 2)
+            acc += (uint64_t)a[K >> 1] * a[K >> 1];   /* top column: fits 64 bits */
+        else if constexpr ((K & 1) == 0)
             mac96(a[K >> 1], a[K >> 1], acc, c2);
         if (K < N - 1)
             st |= (uint32_t)acc;

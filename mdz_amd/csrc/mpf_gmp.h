@@ -186,8 +186,11 @@ GF_HD void word_cols(const uint32_t* UW, const uint32_t* VW, uint32_t* ZW, uint6
             ca[t] = UW[i0 + t];
             cb[t] = VW[Q - i0 - t];
         }
-        uint32_t c2;
-        mdzg::mac_col<n, true>(ca, cb, acc, c2);
+        uint32_t c2 = 0;
+        if constexpr (Q == 2 * W - 2)
+            acc += (uint64_t)ca[0] * cb[0];   /* top column: the sum fits 64 bits */
+        else
+            mdzg::mac_col<n, true>(ca, cb, acc, c2);
         ZW[Q] = (uint32_t)acc;
         acc = (acc >> 32) | ((uint64_t)c2 << 32);
         word_cols<W, Q + 1>(UW, VW, ZW, acc);
