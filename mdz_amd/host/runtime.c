@@ -1039,10 +1039,13 @@ int mdzg_render_tile_noalloc(mdzg_ctx* c, const mdzg_tile* t, mdzg_stats* st)
 /* The per-operation probe of the RN arithmetic (measurement only): the
    tile's view must be on the MPFR path at p = 128 or 1024 (the 128- and
    1024-bit legs, whole words); its pixels give the lanes their orbits. */
-_Static_assert(MDZG_OPP_PRODUCT == MDZG_OPP_MUL && MDZG_OPP_SQUARE == MDZG_OPP_SQR &&
-               MDZG_OPP_ADD_SUB == MDZG_OPP_ADD && MDZG_OPP_SQUARES_DIFF == MDZG_OPP_SUBNN &&
-               MDZG_OPP_ESCAPE == MDZG_OPP_ESC && MDZG_OPP_ITERATION == MDZG_OPP_STEP &&
-               MDZG_OPP_LOOP_ONLY == MDZG_OPP_LOOP, "probe operation numbers");
+_Static_assert((int)MDZG_OPP_PRODUCT == (int)MDZG_OPP_MUL &&
+               (int)MDZG_OPP_SQUARE == (int)MDZG_OPP_SQR &&
+               (int)MDZG_OPP_ADD_SUB == (int)MDZG_OPP_ADD &&
+               (int)MDZG_OPP_SQUARES_DIFF == (int)MDZG_OPP_SUBNN &&
+               (int)MDZG_OPP_ESCAPE == (int)MDZG_OPP_ESC &&
+               (int)MDZG_OPP_ITERATION == (int)MDZG_OPP_STEP &&
+               (int)MDZG_OPP_LOOP_ONLY == (int)MDZG_OPP_LOOP, "probe operation numbers");
 
 int mdzg_op_probe(mdzg_ctx* c, const mdzg_tile* t, int op, int warm, int reps, double* kernel_ms)
 {
