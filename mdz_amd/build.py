@@ -65,6 +65,8 @@ def jobs():
                    "-o", os.path.join(BUILD, "k_colour.o")], "k_colour.o"))
     steps.append(([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "noalloc.hip"),
                    "-o", os.path.join(BUILD, "noalloc.o")], "noalloc.o"))
+    steps.append(([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "k_probe.hip"),
+                   "-o", os.path.join(BUILD, "k_probe.o")], "k_probe.o"))
     steps.append(([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "graph.hip"),
                    "-o", os.path.join(BUILD, "graph.o")], "graph.o"))
     for n in sorted(NWORDS, reverse=True):   # largest (slowest) first

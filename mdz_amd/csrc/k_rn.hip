@@ -1,8 +1,6 @@
 /* k_rn.hip -- the RN<N> kernels of k_escape.h for one word count, selected by
  * -DMDZG_N=<N> at compile time (one object per N, built in parallel).
- * Exports mdzg_kget_rn<N>(full, &tile, &persist, &slice) to the launch table,
- * and for N = 4 and 32 (the 128- and 1024-bit legs) mdzg_kprobe_rn<N>(), the
- * per-operation probe kernel. */
+ * Exports mdzg_kget_rn<N>(full, &tile, &persist, &slice) to the launch table. */
 #include "k_escape.h"
 
 #ifndef MDZG_N
@@ -26,11 +24,3 @@ extern "C" void KGET_NAME(MDZG_N)(int full, const void** tile, const void** pers
     }
 }
 
-#if MDZG_N == 4 || MDZG_N == 32
-#define KPROBE_NAME2(n) mdzg_kprobe_rn##n
-#define KPROBE_NAME(n) KPROBE_NAME2(n)
-extern "C" const void* KPROBE_NAME(MDZG_N)(void)
-{
-    return (const void*)k_rn_op_probe<MDZG_N>;
-}
-#endif

@@ -15,7 +15,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BUILD = os.path.join(ROOT, "mdz_amd", "_build")
-DEVICE_OBJS = ["launch.o", "noalloc.o", "graph.o", "k_fp64.o", "k_gmp.o", "k_colour.o"] + \
+DEVICE_OBJS = ["launch.o", "noalloc.o", "graph.o", "k_fp64.o", "k_gmp.o", "k_colour.o", "k_probe.o"] + \
     [f"k_rn{n}.o" for n in (1, 2, 3, 4, 6, 8, 12, 16, 24, 32)]
 
 
