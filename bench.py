@@ -328,7 +328,12 @@ def serve_run(ctx, M, np, mode, tiles, rate, seed=1, palette=None):
     return dict(rate=rate, n=len(tiles), p50_ms=float(np.percentile(lat, 50)),
                 p99_ms=float(np.percentile(lat, 99)), mean_ms=float(lat.mean()),
                 tiles_per_s=len(tiles) / span, gpix_per_s=pix / span / 1e9,
-                slices=st["slices"], hbm_peak=st["hbm_peak"])
+                slices=st["slices"], hbm_peak=st["hbm_peak"], latencies_ms=lat.tolist())
+
+
+def _brief(r):
+    """a serve_run result without its per-request latencies"""
+    return {k: v for k, v in r.items() if k != "latencies_ms"}
 
 
 def unloaded_latency(ctx, M, np, mode, tiles, reps=15):
@@ -359,7 +364,7 @@ def config4_serving(ctx, M, WL, np, fracs=(0.25, 0.5, 0.75, 0.9), n=300):
             sweep.append(r)
             log(f"  load {f:.2f}: {r['tiles_per_s']:.0f} tiles/s p50 {r['p50_ms']:.2f} "
                 f"p99 {r['p99_ms']:.2f} ms")
-        out[name] = dict(unloaded_ms=unl, saturation=sat, sweep=sweep)
+        out[name] = dict(unloaded_ms=unl, saturation=_brief(sat), sweep=[_brief(x) for x in sweep])
     # the same requests as image requests: counts coloured in HBM, colours returned
     rng = np.random.default_rng(7)
     pal = M.make_palette([int(x) for x in rng.integers(0, 1 << 24, 256)], offset=148,
@@ -367,16 +372,68 @@ def config4_serving(ctx, M, WL, np, fracs=(0.25, 0.5, 0.75, 0.9), n=300):
     img = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, float("inf"), palette=pal)
     log(f"serve batched, image requests: saturation {img['tiles_per_s']:.0f} tiles/s "
         f"{img['gpix_per_s']:.1f} G pixel-iter/s")
-    out["batched_image_saturation"] = img
+    out["batched_image_saturation"] = _brief(img)
     # the batched form with each slice's launches made one by one (no graphs)
     ctx.option_set(M.OPT_GRAPHS, 0)
     unl0 = unloaded_latency(ctx, M, np, M.SERVE_BATCHED, tiles)
     sat0 = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, float("inf"))
     half0 = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, 0.5 * sat0["tiles_per_s"])
     ctx.option_set(M.OPT_GRAPHS, 1)
-    out["batched_without_graphs"] = dict(unloaded_ms=unl0, saturation=sat0, half_load=half0)
+    out["batched_without_graphs"] = dict(unloaded_ms=unl0, saturation=_brief(sat0),
+                                         half_load=_brief(half0))
     log(f"serve batched without graphs: unloaded {unl0:.2f} ms, saturation "
         f"{sat0['gpix_per_s']:.1f} G pixel-iter/s, half load p99 {half0['p99_ms']:.2f} ms")
+    return out
+
+
+def config4_mix(ctx, M, WL, np, n=48, fracs=(0.25, 0.5, 0.75), seed=0x5E4):
+    """Config 4 on tiles drawn from config 3's distribution (sides 64-512,
+    widths 1e-2..1e-250 and so FP64 to 1024-bit tiles, depths 1e3-5e4, three
+    formulas): each request's latency is also taken over its own unloaded
+    latency (the tile alone on the server), since the tiles' costs span four
+    orders of magnitude."""
+    specs = WL.config3_specs(n, seed=seed)
+    tiles = [(WL.spec_view(sp), None, None) for sp in specs]
+    serve_run(ctx, M, np, M.SERVE_BATCHED, tiles[:8], float("inf"))      # warm-up
+    srv = M.Server(ctx, mode=M.SERVE_BATCHED)
+    alone = []
+    for v, l, c in tiles:
+        alone.append(srv.wait(srv.submit(v, l, c))[1]["latency_ms"])
+    srv.close()
+    alone = np.array(alone)
+
+    def slow(r):
+        x = np.array(r["latencies_ms"]) / alone
+        return dict(p50_slowdown=float(np.percentile(x, 50)),
+                    p99_slowdown=float(np.percentile(x, 99)))
+
+    out = dict(workload=f"{n} tiles of config 3's distribution (seed {seed:#x}): "
+                        f"{sum(sp['side'] ** 2 for sp in specs)} pixels, precisions "
+                        f"{sorted(set(sp['precision'] for sp in specs))}",
+               unloaded_ms=dict(p50=float(np.percentile(alone, 50)),
+                                p99=float(np.percentile(alone, 99)), sum=float(alone.sum())))
+    for name, mode in (("batched", M.SERVE_BATCHED), ("per_request", M.SERVE_PER_REQUEST)):
+        sat = serve_run(ctx, M, np, mode, tiles, float("inf"))
+        out[name] = dict(saturation=_brief(sat), **slow(sat))
+        log(f"mix {name}: saturation {sat['tiles_per_s']:.1f} tiles/s "
+            f"{sat['gpix_per_s']:.1f} G pixel-iter/s")
+    sweep = []
+    for f in fracs:
+        r = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, f * out["batched"]["saturation"]["tiles_per_s"])
+        row = _brief(r)
+        row.update(load=f, **slow(r))
+        sweep.append(row)
+        log(f"  mix load {f:.2f}: {r['gpix_per_s']:.1f} G, p50 {r['p50_ms']:.1f} p99 "
+            f"{r['p99_ms']:.1f} ms, p99 slowdown {row['p99_slowdown']:.2f}")
+    out["batched"]["sweep"] = sweep
+    ok = [x for x in sweep if x["p99_slowdown"] <= 2.0]
+    best = max(ok, key=lambda x: x["gpix_per_s"]) if ok else None
+    pr = out["per_request"]["saturation"]["gpix_per_s"]
+    out["north_star"] = dict(load=best["load"] if best else None,
+                             batched_gpix_per_s=best["gpix_per_s"] if best else None,
+                             per_request_saturation_gpix_per_s=pr,
+                             ratio=best["gpix_per_s"] / pr if best else None,
+                             target="ratio >= 3 at p99 slowdown <= 2")
     return out
 
 
@@ -797,6 +854,7 @@ def main():
                    serving=sv,
                    throughput_ratio_batched_vs_per_request=b["saturation"]["gpix_per_s"] /
                    t["saturation"]["gpix_per_s"])
+        res["serving_config3_mix"] = config4_mix(ctx, M, WL, np)
     else:
         raise SystemExit(f"unknown --config {a.config}")
 

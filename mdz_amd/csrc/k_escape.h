@@ -547,11 +547,12 @@ constexpr int rn_min_waves()
 
 /* the slice kernel keeps its resume and save paths live inside the
    iteration loop, so it gets a larger share of the register file than the
-   other kernels from 8 words up (a single wave's whole file from 24) */
+   other kernels from 8 words up: two waves per SIMD (one for the ragged
+   32-word class, as the other kernels) */
 template <int N, bool FULL>
 constexpr int rn_min_waves_slice()
 {
-    return N >= 24 ? 1 : (N >= 8 ? 2 : rn_min_waves<N, FULL>());
+    return (N >= 32 && !FULL) ? 1 : (N >= 8 ? 2 : rn_min_waves<N, FULL>());
 }
 
 /* c per lane: registers for small N; for N >= 24 the block's LDS (word w of
@@ -848,8 +849,14 @@ __global__ __launch_bounds__(256, (rn_min_waves_slice<N, FULL>())) void k_rn_sli
     s.c.init(c_lds);
     bool drained = false;
     wave_run run = { 0, 0u };
+    /* the claim loop runs until the wave holds a pixel or the queue is
+       drained, and the kernel has a single exit: with a `continue` back to
+       the claim instead, the 1024-bit kernel needed more than the 256 VGPRs
+       of two waves per SIMD */
     for (;;) {
-        if (!drained) {
+        do {
+          if (drained)
+              break;
           const bool need = !active;
           const uint64_t g = claim<claim_run<N>()>(need, counter, run);
           if (need && g < total) {
@@ -886,12 +893,9 @@ __global__ __launch_bounds__(256, (rn_min_waves_slice<N, FULL>())) void k_rn_sli
             }
           }
           drained = __any(need && g >= total);
-        }
-        if (!__any(active)) {
-            if (drained)
-                break;
-            continue;
-        }
+        } while (!__any(active));
+        if (!__any(active))
+            break;
         if (active) {
             ++wz;
             const bool esc = rn_step<N, FULL>(s, fractal, wz, kclr);

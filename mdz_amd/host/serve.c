@@ -481,24 +481,39 @@ static void* sched_batched(void* arg)
             pthread_cond_wait(&s->cv_work, &s->lock);
         if (s->shutdown && s->nact == 0 && s->pend_head == s->pend_tail)
             break;
-        /* admission: oldest first, stop at the first request that does not fit */
+        /* admission, oldest first.  A request that does not fit its class's
+           budget of resident pixels holds back the later requests of its
+           class (first come, first served within a class) but not those of
+           other classes: a 1024-bit tile waiting for room does not keep an
+           FP64 tile off the device.  No free slot or no arena block stops
+           admission for this round. */
         int nnew = 0;
-        while (s->pend_head != s->pend_tail) {
-            sreq* r = s->reqs[s->pend[s->pend_head]];
+        int blocked[MAX_KID];
+        memset(blocked, 0, sizeof blocked);
+        size_t keep = s->pend_head;
+        int full = 0;
+        for (size_t pi = s->pend_head; pi < s->pend_tail; ++pi) {
+            const size_t ri = s->pend[pi];
+            sreq* r = s->reqs[ri];
             const int k = r->pv.kid;
-            if (s->act_pix[k] > 0 && s->act_pix[k] + r->npix > s->budget_pix[k])
-                break;
+            if (full || blocked[k] ||
+                (s->act_pix[k] > 0 && s->act_pix[k] + r->npix > s->budget_pix[k])) {
+                blocked[k] = 1;
+                s->pend[keep++] = ri;
+                continue;
+            }
             int slot = -1;
             for (int i = 0; i < s->nslots; ++i)
                 if (!s->slot_used[i]) {
                     slot = i;
                     break;
                 }
-            if (slot < 0)
-                break;
-            size_t off;
-            if (arena_alloc(s, r->bytes, &off) != 0)
-                break;
+            size_t off = 0;
+            if (slot < 0 || arena_alloc(s, r->bytes, &off) != 0) {
+                full = 1;
+                s->pend[keep++] = ri;
+                continue;
+            }
             s->slot_used[slot] = 1;
             r->slot = slot;
             r->off = off;
@@ -515,9 +530,9 @@ static void* sched_batched(void* arg)
                 fresh = (sreq**)realloc(fresh, sizeof(sreq*) * (size_t)capfresh);
             }
             fresh[nnew++] = r;
-            s->pend_head++;
             s->st.admitted++;
         }
+        s->pend_tail = keep;
         pthread_mutex_unlock(&s->lock);
 
         int rc = MDZG_OK;
