@@ -417,15 +417,19 @@ def config4_mix(ctx, M, WL, np, n=48, fracs=(0.25, 0.5, 0.75), seed=0x5E4):
         out[name] = dict(saturation=_brief(sat), **slow(sat))
         log(f"mix {name}: saturation {sat['tiles_per_s']:.1f} tiles/s "
             f"{sat['gpix_per_s']:.1f} G pixel-iter/s")
-    sweep = []
-    for f in fracs:
-        r = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, f * out["batched"]["saturation"]["tiles_per_s"])
-        row = _brief(r)
-        row.update(load=f, **slow(r))
-        sweep.append(row)
-        log(f"  mix load {f:.2f}: {r['gpix_per_s']:.1f} G, p50 {r['p50_ms']:.1f} p99 "
-            f"{r['p99_ms']:.1f} ms, p99 slowdown {row['p99_slowdown']:.2f}")
-    out["batched"]["sweep"] = sweep
+    for name, mode in (("batched", M.SERVE_BATCHED), ("per_request", M.SERVE_PER_REQUEST)):
+        sweep = []
+        for f in fracs:
+            # the same arrival rates for both forms: fractions of the batched saturation
+            r = serve_run(ctx, M, np, mode, tiles, f * out["batched"]["saturation"]["tiles_per_s"])
+            row = _brief(r)
+            row.update(load=f, **slow(r))
+            sweep.append(row)
+            log(f"  mix {name} load {f:.2f}: {r['gpix_per_s']:.1f} G, p50 {r['p50_ms']:.1f} p99 "
+                f"{r['p99_ms']:.1f} ms, p50/p99 slowdown {row['p50_slowdown']:.2f}/"
+                f"{row['p99_slowdown']:.2f}")
+        out[name]["sweep"] = sweep
+    sweep = out["batched"]["sweep"]
     ok = [x for x in sweep if x["p99_slowdown"] <= 2.0]
     best = max(ok, key=lambda x: x["gpix_per_s"]) if ok else None
     pr = out["per_request"]["saturation"]["gpix_per_s"]
