@@ -452,16 +452,25 @@ __global__ __launch_bounds__(256) void k_fp64_persist(const mdzg_dtile* tiles, i
     }
 }
 
-/* Resumable slice (serving): every unfinished pixel of every tile advances by at
-   most `budget` iterations; finished pixels write their count and decrement
-   their request's remaining counter, the others save z, z^2 and wz. */
+/* Resumable slice (serving).  Every lane has `budget` iterations for the
+   slice.  It claims the tiles' unfinished pixels in queue order (the server
+   puts the requests with the least work left first) and runs each until it
+   finishes -- its count written, its request's remaining counter
+   decremented -- or until the lane's budget is spent, when it saves z, z^2
+   and wz.  Pixels the slice does not reach keep their state for the next.
+   A slice therefore lasts about `budget` iterations however many pixels are
+   active, and a small request at the front of the queue advances as fast as
+   it would alone. */
 __global__ __launch_bounds__(256) void k_fp64_slice(const mdzg_dtile* tiles, int ntiles,
                                                     uint64_t total,
-                                                    unsigned long long* counter, int budget)
+                                                    unsigned long long* counter, int budget,
+    int lane_budget)
 {
     bool active = false;
     int fractal = 0;
-    int32_t wz = 0, depth = 0, left = 0;
+    int32_t wz = 0, depth = 0;
+    int32_t left = 0;           /* this pixel's iterations left in the slice */
+    int32_t lb = lane_budget;   /* this lane's iterations left in the slice */
     uint32_t local = 0;
     const mdzg_dtile* tp = tiles;
     fp64_state s;
@@ -470,7 +479,7 @@ __global__ __launch_bounds__(256) void k_fp64_slice(const mdzg_dtile* tiles, int
     wave_run run = { 0, 0u };
     for (;;) {
         if (!drained) {
-          const bool need = !active;
+          const bool need = !active && lb > 0;
           const uint64_t g = claim<64u>(need, counter, run);
           if (need && g < total) {
             const int ti = find_tile(tiles, ntiles, g);
@@ -494,7 +503,7 @@ __global__ __launch_bounds__(256) void k_fp64_slice(const mdzg_dtile* tiles, int
                 active = true;
             }
           }
-          drained = __any(need && g >= total);
+          drained = __any(need && g >= total) || !__any(lb > 0);
         }
         if (!__any(active)) {
             if (drained)
@@ -504,12 +513,14 @@ __global__ __launch_bounds__(256) void k_fp64_slice(const mdzg_dtile* tiles, int
         for (int it = 0; it < 32; ++it) {
             if (active) {
                 ++wz;
+                --left;
+                --lb;
                 const bool esc = fp64_step(s, fractal, wz);
                 if (esc || wz >= depth) {
                     put_count(tp->out + local, esc ? (uint32_t)wz : 0u);
                     atomicSub(tp->remaining, 1u);
                     active = false;
-                } else if (--left == 0) {
+                } else if (left == 0 || lb == 0) {
                     double* st = (double*)tp->state + 4 * (size_t)local;
                     st[0] = s.wre;
                     st[1] = s.wim;
@@ -837,11 +848,14 @@ __device__ __forceinline__ void store_mp(uint32_t* p, const mp<N>& v)
 /* Resumable slice (serving), RN arithmetic: see k_fp64_slice. */
 template <int N, bool FULL>
 __global__ __launch_bounds__(256, (rn_min_waves_slice<N, FULL>())) void k_rn_slice(
-    const mdzg_dtile* tiles, int ntiles, uint64_t total, unsigned long long* counter, int budget)
+    const mdzg_dtile* tiles, int ntiles, uint64_t total, unsigned long long* counter, int budget,
+    int lane_budget)
 {
     bool active = false;
     int fractal = 0, kclr = 0;
-    int32_t wz = 0, depth = 0, left = 0;
+    int32_t wz = 0, depth = 0;
+    int32_t left = 0;           /* this pixel's iterations left in the slice */
+    int32_t lb = lane_budget;   /* this lane's iterations left in the slice */
     uint32_t local = 0;
     const mdzg_dtile* tp = tiles;
     __shared__ uint32_t c_lds[c_in_lds<N>() ? 2 * (N + 2) * 256 : 1];
@@ -857,7 +871,7 @@ __global__ __launch_bounds__(256, (rn_min_waves_slice<N, FULL>())) void k_rn_sli
         do {
           if (drained)
               break;
-          const bool need = !active;
+          const bool need = !active && lb > 0;
           const uint64_t g = claim<claim_run<N>()>(need, counter, run);
           if (need && g < total) {
             const int ti = find_tile(tiles, ntiles, g);
@@ -892,18 +906,20 @@ __global__ __launch_bounds__(256, (rn_min_waves_slice<N, FULL>())) void k_rn_sli
                 active = true;
             }
           }
-          drained = __any(need && g >= total);
+          drained = __any(need && g >= total) || !__any(lb > 0);
         } while (!__any(active));
         if (!__any(active))
             break;
         if (active) {
             ++wz;
+            --left;
+            --lb;
             const bool esc = rn_step<N, FULL>(s, fractal, wz, kclr);
             if (esc || wz >= depth) {
                 put_count(tp->out + local, esc ? (uint32_t)wz : 0u);
                 atomicSub(tp->remaining, 1u);
                 active = false;
-            } else if (--left == 0) {
+            } else if (left == 0 || lb == 0) {
                 uint32_t* st = tp->state + (size_t)local * 4 * MDZG_VS(N);
                 store_mp<N>(st, s.wre);
                 store_mp<N>(st + MDZG_VS(N), s.wim);

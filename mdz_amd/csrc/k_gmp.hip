@@ -180,11 +180,14 @@ __global__ __launch_bounds__(256) void k_gmp_persist(const mdzg_dtile* tiles, in
 template <int L>
 __global__ __launch_bounds__(256) void k_gmp_slice(const mdzg_dtile* tiles, int ntiles,
                                                    uint64_t total, unsigned long long* counter,
-                                                   int budget)
+                                                   int budget,
+    int lane_budget)
 {
     bool active = false;
     int fractal = 0, lr = 0;
-    int32_t wz = 0, depth = 0, left = 0;
+    int32_t wz = 0, depth = 0;
+    int32_t left = 0;           /* this pixel's iterations left in the slice */
+    int32_t lb = lane_budget;   /* this lane's iterations left in the slice */
     uint32_t local = 0;
     const mdzg_dtile* tp = tiles;
     gmp_state<L> s;
@@ -192,7 +195,7 @@ __global__ __launch_bounds__(256) void k_gmp_slice(const mdzg_dtile* tiles, int 
     wave_run run = { 0, 0u };
     for (;;) {
         if (!drained) {
-            const bool need = !active;
+            const bool need = !active && lb > 0;
             const uint64_t g = claim<0u>(need, counter, run);
             if (need && g < total) {
                 const int ti = find_tile(tiles, ntiles, g);
@@ -226,7 +229,7 @@ __global__ __launch_bounds__(256) void k_gmp_slice(const mdzg_dtile* tiles, int 
                     active = true;
                 }
             }
-            drained = __any(need && g >= total);
+            drained = __any(need && g >= total) || !__any(lb > 0);
         }
         if (!__any(active)) {
             if (drained)
@@ -235,12 +238,14 @@ __global__ __launch_bounds__(256) void k_gmp_slice(const mdzg_dtile* tiles, int 
         }
         if (active) {
             ++wz;
+            --left;
+            --lb;
             const bool esc = gmp_step<L>(s, fractal, wz, lr);
             if (esc || wz >= depth) {
                 put_count(tp->out + local, esc ? (uint32_t)wz : 0u);
                 atomicSub(tp->remaining, 1u);
                 active = false;
-            } else if (--left == 0) {
+            } else if (left == 0 || lb == 0) {
                 const int vs = tp->vs;
                 uint32_t* st = tp->state + (size_t)local * 4 * vs;
                 store_gf<L>(st, s.wre);

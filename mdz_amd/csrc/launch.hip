@@ -184,9 +184,11 @@ int mdzg_dev_node_persist(mdzg_gnode* n, int kid, const mdzg_dtile* d_tiles, int
 }
 
 int mdzg_dev_node_slice(mdzg_gnode* n, int kid, const mdzg_dtile* d_tiles, int ntiles,
-                        uint64_t total, unsigned long long* d_counter, int grid, int budget)
+                        uint64_t total, unsigned long long* d_counter, int grid, int budget,
+                        int lane_budget)
 {
-    if (!n || kid < 0 || kid >= mdzg_dev_nkernels() || ntiles <= 0 || grid <= 0 || budget <= 0)
+    if (!n || kid < 0 || kid >= mdzg_dev_nkernels() || ntiles <= 0 || grid <= 0 || budget <= 0 ||
+        lane_budget <= 0)
         return -1;
     const void* f;
     lookup(kid, 2, &f);
@@ -196,6 +198,7 @@ int mdzg_dev_node_slice(mdzg_gnode* n, int kid, const mdzg_dtile* d_tiles, int n
     node_arg(n, total);
     node_arg(n, d_counter);
     node_arg(n, budget);
+    node_arg(n, lane_budget);
     return 0;
 }
 
@@ -261,14 +264,16 @@ int mdzg_dev_occupancy_slice(int kid)
 }
 
 int mdzg_dev_launch_slice(int kid, const mdzg_dtile* d_tiles, int ntiles, uint64_t total,
-                          unsigned long long* d_counter, int grid, int budget, hipStream_t st)
+                          unsigned long long* d_counter, int grid, int budget, int lane_budget,
+                          hipStream_t st)
 {
-    if (kid < 0 || kid >= mdzg_dev_nkernels() || ntiles <= 0 || grid <= 0 || budget <= 0)
+    if (kid < 0 || kid >= mdzg_dev_nkernels() || ntiles <= 0 || grid <= 0 || budget <= 0 ||
+        lane_budget <= 0)
         return -1;
     const void* f;
     lookup(kid, 2, &f);
     void* args[] = { (void*)&d_tiles, (void*)&ntiles, (void*)&total, (void*)&d_counter,
-                     (void*)&budget };
+                     (void*)&budget, (void*)&lane_budget };
     return hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, st) == hipSuccess ? 0 : -3;
 }
 

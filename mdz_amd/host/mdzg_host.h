@@ -32,7 +32,8 @@ int mdzg_dev_node_persist(mdzg_gnode* n, int kid, const mdzg_dtile* d_tiles, int
                           uint64_t total, unsigned long long* d_counter, const int* d_cancel,
                           int grid, int chunk);
 int mdzg_dev_node_slice(mdzg_gnode* n, int kid, const mdzg_dtile* d_tiles, int ntiles,
-                        uint64_t total, unsigned long long* d_counter, int grid, int budget);
+                        uint64_t total, unsigned long long* d_counter, int grid, int budget,
+                        int lane_budget);
 int mdzg_dev_node_colour(mdzg_gnode* node, const uint32_t* counts, int real_w, int user_w,
                          int user_h, int aa, const uint32_t* palette, int n, int offset,
                          double sc, int ip, uint32_t depth, uint32_t* rgb,
@@ -45,7 +46,8 @@ void mdzg_gcache_counts(mdzg_gcache* gc, uint64_t* hits, uint64_t* builds);
 int mdzg_graph_run(mdzg_gcache* gc, const mdzg_gnode* root, const mdzg_gnode* par, int npar,
                    const mdzg_gnode* tail, hipStream_t st);
 int mdzg_dev_launch_slice(int kid, const mdzg_dtile* d_tiles, int ntiles, uint64_t total,
-                          unsigned long long* d_counter, int grid, int budget, hipStream_t st);
+                          unsigned long long* d_counter, int grid, int budget, int lane_budget,
+                          hipStream_t st);
 
 /* k_colour.hip: counts -> RGB (palette_apply / do_anti_aliasing), adding the
    pixel-iterations of the counts read to *iters when iters is not null; at

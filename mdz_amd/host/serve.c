@@ -50,6 +50,7 @@ typedef struct sreq {
     int status;                 /* 0 pending, 1 active, 2 done, < 0 error */
     double t_submit, t_admit, t_done;
     double pixel_iters;
+    double work_left;           /* unfinished pixels x depth: the slice queue's order */
     mdzg_job* job;              /* per-request mode */
     /* device addresses (batched mode) */
     uint32_t *d_xs, *d_ys, *d_jc, *d_out, *d_state;
@@ -349,14 +350,40 @@ static int run_slice(mdzg_server* s)
     int first[MAX_KID], cnt[MAX_KID];
     uint64_t total[MAX_KID];
     int n = 0;
+    /* Each class's requests go in the slice queue with the least work left
+       first (oldest first among equals).  A class whose requests are alike
+       gives every pixel slice_iters iterations a slice.  A class whose
+       requests differ in work by more than 8x caps each lane at
+       2 x slice_iters iterations instead and lets a pixel run on within
+       that: the lanes claim from the front, so a small request advances as
+       it would alone instead of sharing every slice with a large one, and
+       the large one takes the rest of the lanes. */
+    sreq** order = (sreq**)malloc(sizeof(sreq*) * (size_t)(s->nact > 0 ? s->nact : 1));
+    if (!order)
+        return MDZG_ERR_NOMEM;
+    int lane_cap[MAX_KID];
     for (int k = 0; k < MAX_KID; ++k) {
         first[k] = n;
         cnt[k] = 0;
         total[k] = 0;
+        lane_cap[k] = 0;
+        int m = 0;
         for (int i = 0; i < s->nact; ++i) {
             sreq* r = s->act[i];
             if (r->pv.kid != k || r->status < 0)
                 continue;
+            int j = m++;
+            while (j > 0 && (order[j - 1]->work_left > r->work_left ||
+                             (order[j - 1]->work_left == r->work_left && order[j - 1]->id > r->id))) {
+                order[j] = order[j - 1];
+                --j;
+            }
+            order[j] = r;
+        }
+        if (m >= 2 && order[m - 1]->work_left > 8.0 * order[0]->work_left)
+            lane_cap[k] = 1;
+        for (int oi = 0; oi < m; ++oi) {
+            sreq* r = order[oi];
             mdzg_dtile* d = &s->h_desc[n++];
             memset(d, 0, sizeof *d);
             d->out = r->d_out;
@@ -387,6 +414,7 @@ static int run_slice(mdzg_server* s)
             cnt[k]++;
         }
     }
+    free(order);
     if (n == 0)
         return MDZG_OK;          /* every active request failed its staging */
     if (hipMemcpyAsync(s->d_desc, s->h_desc, sizeof(mdzg_dtile) * (size_t)n,
@@ -406,8 +434,10 @@ static int run_slice(mdzg_server* s)
         const uint64_t want = (total[k] + 255) / 256;
         int grid = (int)(want < cap ? want : cap);
         if (grid < 1) grid = 1;
+        const int it = s->slice_iters[k];
         if (mdzg_dev_node_slice(&par[nk++], k, s->d_desc + first[k], cnt[k], total[k],
-                                s->d_ctr + k, grid, s->slice_iters[k]) != 0)
+                                s->d_ctr + k, grid, lane_cap[k] ? INT32_MAX : it,
+                                lane_cap[k] ? 2 * it : INT32_MAX) != 0)
             return MDZG_ERR_DEVICE;
     }
     if (c->graphs && nk >= 2) {
@@ -518,6 +548,7 @@ static void* sched_batched(void* arg)
             r->slot = slot;
             r->off = off;
             r->status = 1;
+            r->work_left = (double)r->npix * (double)r->tile.view.depth;
             r->t_admit = mdzg_now_ms();
             s->act_pix[k] += r->npix;
             if (s->nact == s->capact) {
@@ -561,6 +592,7 @@ static void* sched_batched(void* arg)
             sreq* r = s->act[i];
             const int done = (rc != MDZG_OK) || r->status < 0 || s->h_remaining[r->slot] == 0;
             if (!done) {
+                r->work_left = (double)s->h_remaining[r->slot] * (double)r->tile.view.depth;
                 ++i;
                 continue;
             }

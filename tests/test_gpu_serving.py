@@ -56,6 +56,26 @@ def test_server_results_equal_standalone(ctx, slice_iters):
         assert _eq(out, ref)
 
 
+def test_server_unequal_requests_in_one_class(ctx):
+    """Requests of one class whose work differs by far more than 8x: the
+    slice caps each lane's iterations and lets pixels run on within the cap
+    (the queue ordered by work left).  Counts equal standalone renders,
+    across many slices, with the large request arriving first and last."""
+    v = WL.config2_view()
+    ve = WL.config2_escaping_view()
+    big = (v, (200, 96), (300, 96))
+    small = [(ve, (40 + 70 * i, 8), (100 + 90 * i, 8)) for i in range(4)]
+    for reqs in ([big] + small, small + [big]):
+        alone = [ctx.render(x, lines=l, cols=c)[0] for x, l, c in reqs]
+        with M.Server(ctx, slice_iters=50) as srv:
+            ids = [srv.submit(x, l, c) for x, l, c in reqs]
+            outs = [srv.wait(i) for i in ids]
+            assert srv.stats()["slices"] > 20
+        for (out, info), ref in zip(outs, alone):
+            assert info["status"] == 2
+            assert _eq(out, ref)
+
+
 def test_server_tight_hbm_budget(ctx):
     """Only a few requests fit at once: admission queues the rest, results hold."""
     reqs = _mix() * 2
