@@ -290,11 +290,26 @@ GF_HD void gf_mul(gf<L>& r, const gf<L>& u, const gf<L>& v, int lr)
     const int P = lr - 1;
     const int us = u.size < P ? u.size : P;
     const int vs = v.size < P ? v.size : P;
+    /* Limbs beyond a value's size are zero.  So when every lane of the wave
+       runs at the bucket's own PREC + 1 = L limbs, the cut to PREC limbs
+       only clears the last limb, and the product's limbs from us + vs on are
+       zero, which leaves the result window without selects. */
+    const bool full = !gf_any(lr != L);
     limb U[L], V[L];
+    if (full) {
 #pragma unroll
-    for (int i = 0; i < L; ++i) {
-        U[i] = (i < us) ? u.t[i] : (limb)0;
-        V[i] = (i < vs) ? v.t[i] : (limb)0;
+        for (int i = 0; i < L - 1; ++i) {
+            U[i] = u.t[i];
+            V[i] = v.t[i];
+        }
+        U[L - 1] = 0;
+        V[L - 1] = 0;
+    } else {
+#pragma unroll
+        for (int i = 0; i < L; ++i) {
+            U[i] = (i < us) ? u.t[i] : (limb)0;
+            V[i] = (i < vs) ? v.t[i] : (limb)0;
+        }
     }
     /* U and V hold at most PREC <= L - 1 limbs, so their last limb is zero;
        the product of their top K limbs is formed by word_product.  K = L - 1
@@ -311,8 +326,13 @@ GF_HD void gf_mul(gf<L>& r, const gf<L>& u, const gf<L>& v, int lr)
     const int adj = (Z[0] == 0) ? 1 : 0;
     const int rsize = us + vs - adj;
     const int keep = zero ? 0 : (rsize < lr ? rsize : lr);
+    if (full) {     /* a zero operand leaves Z zero: no case for it either */
 #pragma unroll
-    for (int k = 0; k < L; ++k) r.t[k] = (k < keep) ? (adj ? Z[k + 1] : Z[k]) : (limb)0;
+        for (int k = 0; k < L; ++k) r.t[k] = adj ? Z[k + 1] : Z[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < L; ++k) r.t[k] = (k < keep) ? (adj ? Z[k + 1] : Z[k]) : (limb)0;
+    }
     r.size = keep;
     r.exp = zero ? 0 : u.exp + v.exp - adj;
     r.neg = zero ? 0 : (u.neg ^ v.neg);
@@ -335,18 +355,32 @@ GF_HD void gf_mul2(gf<L>& r, const gf<L>& u, int lr)
 {
     const int P = lr - 1;
     int size = u.size;
-    limb cin = 0;
-    if (size > P) {
-        cin = pick<L>(u.t, P) >> 63;
-        size = P;
-    }
     limb tmp[L];
-    limb c = cin;
+    limb c;
+    if (!gf_any(lr != L)) {
+        /* PREC = L - 1 on every lane: the cut drops t[L-1] (zero unless
+           size = L), and zero limbs shift in as zeros with no carry */
+        size = size < P ? size : P;
+        c = u.t[L - 1] >> 63;
+        tmp[L - 1] = 0;
 #pragma unroll
-    for (int i = L - 1; i >= 0; --i) {
-        const bool in = i < size;
-        tmp[i] = in ? ((u.t[i] << 1) | c) : (limb)0;
-        c = in ? (u.t[i] >> 63) : c;
+        for (int i = L - 2; i >= 0; --i) {
+            tmp[i] = (u.t[i] << 1) | c;
+            c = u.t[i] >> 63;
+        }
+    } else {
+        limb cin = 0;
+        if (size > P) {
+            cin = pick<L>(u.t, P) >> 63;
+            size = P;
+        }
+        c = cin;
+#pragma unroll
+        for (int i = L - 1; i >= 0; --i) {
+            const bool in = i < size;
+            tmp[i] = in ? ((u.t[i] << 1) | c) : (limb)0;
+            c = in ? (u.t[i] >> 63) : c;
+        }
     }
     const bool grow = c != 0;
 #pragma unroll
@@ -688,7 +722,11 @@ GF_HD void gf_addsub(gf<L>& r, const gf<L>& a, const gf<L>& b, uint32_t flipb, i
     const int lim = sub ? lr : lr - 1;
     if (us > lim) us = lim;
     if (vs + ediff > lim) vs = lim - ediff;      /* <= 0: V drops out */
-    keep_top<L>(U, us);
+    if (!gf_any(lr != L))
+        U[L - 1] = (us < L) ? (limb)0 : U[L - 1];  /* zero beyond the size: only
+                                                     a cut to L - 1 clears */
+    else
+        keep_top<L>(U, us);
     keep_top<L>(V, vs);
     int ulast = 0, vlast = 0;                    /* the windows without low zero limbs */
 #pragma unroll

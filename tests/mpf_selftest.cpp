@@ -50,9 +50,13 @@ static void from_gmp(gf<L>& a, const mpf_t x)
     a.neg = x->_mp_size < 0;
 }
 
+// equal values, each with zero limbs beyond its size (the representation
+// the device's full-width forms rely on)
 template <int L>
 static bool same(const gf<L>& a, const gf<L>& b)
 {
+    for (int i = 0; i < L; ++i)
+        if ((i >= a.size && a.t[i] != 0) || (i >= b.size && b.t[i] != 0)) return false;
     if (a.size != b.size) return false;
     if (a.size == 0) return true;
     if (a.exp != b.exp || a.neg != b.neg) return false;
