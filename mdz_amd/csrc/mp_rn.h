@@ -189,7 +189,7 @@ MP_HD void add_ulp(uint32_t* R, int32_t& e, uint32_t inc_word0)
 {
     uint32_t c;
     R[0] = add_cc(R[0], inc_word0, 0u, c);
-    if (wave_any(c != 0)) {          /* R[0] wrapped: rare */
+    if (__builtin_expect(wave_any(c != 0), 0)) {      /* R[0] wrapped: rare */
 #pragma unroll
         for (int j = 1; j < N; ++j) R[j] = add_cc(R[j], 0u, c, c);
         if (c) {
@@ -212,7 +212,7 @@ MP_HD void add_one_if(uint32_t* R, int32_t& e, bool inc)
     asm("v_addc_co_u32 %0, %1, 0, %2, %3" : "=v"(r0), "=s"(co) : "v"(R[0]), "s"(m));
     const uint32_t old = R[0];
     R[0] = r0;
-    if (co != 0) {                    /* R[0] wrapped on some lane: rare */
+    if (__builtin_expect(co != 0, 0)) {       /* R[0] wrapped on some lane: rare */
         uint32_t c = (uint32_t)(r0 < old);
 #pragma unroll
         for (int j = 1; j < N; ++j) R[j] = add_cc(R[j], 0u, c, c);
@@ -574,14 +574,82 @@ MP_HD void mant_mul(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t&
     S = (uint32_t)(st != 0);
 }
 
-/* columns k = K .. 2N-2 of a * a with each off-diagonal product formed once
-   and the column's off-diagonal sum doubled */
+/* Words k = K .. 2N-1 of a * a with each off-diagonal product formed once.
+   Column k of the off-diagonal sum O = sum over i < j of a[i] a[j]
+   2^(32(i+j)) is product-scanned as in mul_cols; as soon as its word o is
+   final, word k of the square, 2 O + sum of a[i]^2 2^(64 i), follows from o,
+   the previous word of O (the doubling shifts its top bit in), one word of
+   a diagonal square and the carry c of the pass: two instructions a word
+   where doubling each column's 96-bit sum took six.  The columns below the
+   first K are left out of both sums, as in mul_cols (a diagonal square whose
+   column 2i is left out is left out whole). */
 template <int N, int K>
-MP_HD void sqr_cols(const uint32_t* a, uint32_t* H, uint64_t& acc, uint32_t& st)
+MP_HD void sqr_cols(const uint32_t* a, uint32_t* H, uint64_t& acc, uint32_t& prev,
+                    uint32_t& c, uint32_t& dhi, uint32_t& st)
+{
+    if constexpr (K < 2 * N) {
+        uint32_t o;
+        if constexpr (K <= 2 * N - 3) {
+            constexpr int i0 = K < N ? 0 : K - N + 1;
+            constexpr int i1 = (K + 1) / 2 - 1;     /* largest i with i < K - i */
+            constexpr int n = i1 >= i0 ? i1 - i0 + 1 : 0;
+            uint32_t c2 = 0;
+            if constexpr (n > 0) {
+                uint32_t ca[n], cb[n];
+#pragma unroll
+                for (int t = 0; t < n; ++t) {
+                    ca[t] = a[i0 + t];
+                    cb[t] = a[K - i0 - t];
+                }
+                mac_col<n, true>(ca, cb, acc, c2);
+            }
+            o = (uint32_t)acc;
+            acc = (acc >> 32) | ((uint64_t)c2 << 32);
+        } else {
+            /* no off-diagonal products: O's last two words (O < 2^(64N-1)) */
+            o = (K == 2 * N - 2) ? (uint32_t)acc : (uint32_t)(acc >> 32);
+        }
+        uint32_t d;
+        if constexpr ((K & 1) == 0) {
+            const uint64_t sq = (uint64_t)a[K >> 1] * a[K >> 1];
+            d = (uint32_t)sq;
+            dhi = (uint32_t)(sq >> 32);
+        } else {
+            d = dhi;
+        }
+        const uint32_t w = add_cc(funnel_r(o, prev, 31u), d, c, c);
+        if (K < N - 1)
+            st |= w;
+        else
+            H[K - (N - 1)] = w;
+        prev = o;
+        sqr_cols<N, K + 1>(a, H, acc, prev, c, dhi, st);
+    }
+}
+
+/* Square with the off-diagonal products formed once (n(n-1)/2
+   multiply-accumulates plus n squares); pays off from N >= 12 words. */
+template <int N, int K0>
+MP_HD void mant_sqr_sym(const uint32_t* a, uint32_t* H, uint32_t& S)
+{
+    uint64_t acc = 0;
+    uint32_t prev = 0, c = 0, dhi = 0, st = 0;
+    sqr_cols<N, K0>(a, H, acc, prev, c, dhi, st);
+    S = (uint32_t)(st != 0);
+}
+
+/* The exact square (all columns), for the rare fallback of the short one:
+   each column's off-diagonal sum doubled into the running column sum.  The
+   same result as the doubling pass above, which needs more registers when
+   it runs over all 2N columns: with it here the ragged 16- and 24-word
+   persistent kernels spilled 260 and 316 bytes a lane (this form: 0 and
+   36). */
+template <int N, int K>
+MP_HD void sqr_cols_exact(const uint32_t* a, uint32_t* H, uint64_t& acc, uint32_t& st)
 {
     if constexpr (K < 2 * N - 1) {
         constexpr int i0 = K < N ? 0 : K - N + 1;
-        constexpr int i1 = (K - 1) / 2;             /* largest i with i < K - i */
+        constexpr int i1 = (K + 1) / 2 - 1;         /* largest i with i < K - i */
         constexpr int n = i1 >= i0 ? i1 - i0 + 1 : 0;
         uint32_t c2 = 0;
         if constexpr (n > 0) {
@@ -613,18 +681,16 @@ MP_HD void sqr_cols(const uint32_t* a, uint32_t* H, uint64_t& acc, uint32_t& st)
         else
             H[K - (N - 1)] = (uint32_t)acc;
         acc = (acc >> 32) | ((uint64_t)c2 << 32);
-        sqr_cols<N, K + 1>(a, H, acc, st);
+        sqr_cols_exact<N, K + 1>(a, H, acc, st);
     }
 }
 
-/* Square with the off-diagonal products formed once and doubled per column
-   (n(n+1)/2 multiply-accumulates); pays off from N >= 12 words. */
-template <int N, int K0>
-MP_HD void mant_sqr_sym(const uint32_t* a, uint32_t* H, uint32_t& S)
+template <int N>
+MP_HD void mant_sqr_exact(const uint32_t* a, uint32_t* H, uint32_t& S)
 {
     uint64_t acc = 0;
     uint32_t st = 0;
-    sqr_cols<N, K0>(a, H, acc, st);
+    sqr_cols_exact<N, 0>(a, H, acc, st);
     H[N] = (uint32_t)acc;
     S = (uint32_t)(st != 0);
 }
@@ -661,7 +727,9 @@ constexpr int short_col()
 template <int N, bool SQR, int K0>
 MP_HD void mant_product(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t& S)
 {
-    if (SQR && N >= 12)
+    if (SQR && N >= 12 && K0 == 0)
+        mant_sqr_exact<N>(a, H, S);
+    else if (SQR && N >= 12)
         mant_sqr_sym<N, K0>(a, H, S);
     else
         mant_mul<N, K0>(a, SQR ? a : b, H, S);
@@ -699,7 +767,13 @@ MP_HD void mul_rn(mp<N>& r, const mp<N>& a, const mp<N>& b, int kclr)
         const bool unsure = (kclr == 0) ? ((uint32_t)(0x80000000u - H[0]) < U)
                                         : ((uint32_t)(H[0] + (U - 1u)) < U);
         S = 1;
-        if (wave_any(unsure)) {
+        /* marked unlikely, the exact product is laid out past the kernel's
+           loops, which keeps the loop's instructions contiguous.  Placed
+           between them (the compiler's choice without the mark), the
+           exact products left the 1024-bit loop with 1.2e8
+           instruction-cache misses on a 737,280-pixel band against 6e4
+           out of line, and 15 % slower. */
+        if (__builtin_expect(wave_any(unsure), 0)) {
             MDZG_NOTE_SHORT_FALLBACK();
             mant_product<N, SQR, 0>(a.m, b.m, H, S);
             e = e0;
@@ -824,7 +898,7 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
     /* cancellation of the whole top word (a subtraction with d <= 1, which
        is exact: S == 0): word shift */
     bool zero = false;
-    if (wave_any(T[N] == 0)) {
+    if (__builtin_expect(wave_any(T[N] == 0), 0)) {
         uint32_t lzw = 0;
         bool found = false;
 #pragma unroll
@@ -909,7 +983,7 @@ MP_HD bool escaped(const mp<N>& wim2, const mp<N>& wre2, double thresh, int kclr
     const double dlt = sa - thresh;
     bool esc = dlt > 0.0;
     const bool unsure = fabs(dlt) <= 0x1p-26;
-    if (wave_any(unsure)) {
+    if (__builtin_expect(wave_any(unsure), 0)) {     /* out of line, as in mul_rn */
         mp<N> t;
         add_rn<N>(t, wim2, wre2, 0u, kclr);
         esc = unsure ? gt4<N>(t) : esc;

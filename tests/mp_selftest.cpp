@@ -319,6 +319,37 @@ static void check_escape(int p, long cases)
     mpfr_clears(A, B, R, (mpfr_ptr)0);
 }
 
+// the symmetric squares (off-diagonal products once, then one doubling pass
+// with the diagonal squares; for the exact square also the per-column
+// doubling form) against the plain product of a with itself: every output
+// word, from column K0 up (K0 = 0, the full product, and the short
+// products' first columns, odd ones included)
+template <int N, int K0>
+static void check_sqr_sym(long cases)
+{
+    for (long it = 0; it < cases; ++it) {
+        uint32_t a[N];
+        for (int j = 0; j < N; ++j) {
+            const uint32_t r = rnd32();
+            a[j] = (r % 5 == 0) ? 0u : (r % 5 == 1) ? ~0u : rnd32();
+        }
+        uint32_t H1[N + 1], H2[N + 1], H3[N + 1], S1, S2, S3;
+        mant_sqr_sym<N, K0>(a, H1, S1);
+        mant_mul<N, K0>(a, a, H2, S2);
+        bool ok = S1 == S2;
+        for (int j = 0; j <= N; ++j) ok = ok && H1[j] == H2[j];
+        if (K0 == 0) {              // the exact fallback's per-column form
+            mant_sqr_exact<N>(a, H3, S3);
+            ok = ok && S3 == S2;
+            for (int j = 0; j <= N; ++j) ok = ok && H3[j] == H2[j];
+        }
+        if (!ok) {
+            if (fails < 10) fprintf(stderr, "SQR-SYM MISMATCH N=%d K0=%d\n", N, K0);
+            ++fails;
+        }
+    }
+}
+
 template <int N>
 static void check_n(long cases)
 {
@@ -342,6 +373,17 @@ int main(int argc, char** argv)
 {
     rng_state = argc > 1 ? strtoull(argv[1], 0, 0) : 1;
     long cases = argc > 2 ? atol(argv[2]) : 2000;
+    check_sqr_sym<1, 0>(cases);
+    check_sqr_sym<2, 0>(cases);
+    check_sqr_sym<2, 1>(cases);
+    check_sqr_sym<3, 0>(cases);
+    check_sqr_sym<5, 3>(cases);
+    check_sqr_sym<12, 0>(cases);
+    check_sqr_sym<12, 10>(cases);
+    check_sqr_sym<16, 14>(cases);
+    check_sqr_sym<24, 22>(cases);
+    check_sqr_sym<32, 0>(cases);
+    check_sqr_sym<32, 30>(cases);
     check_n<1>(cases);
     check_n<2>(cases);
     check_n<3>(cases);
