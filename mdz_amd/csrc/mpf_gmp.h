@@ -761,7 +761,7 @@ GF_HD void gf_addsub(gf<L>& r, const gf<L>& a, const gf<L>& b, uint32_t flipb, i
         exp -= nrm ? 1 : 0;
     }
     uint32_t neg = a.neg ^ ((sub & sw) ? 1u : 0u);
-    if (gf_any(!fast)) {
+    if (__builtin_expect(gf_any(!fast), 0)) {    /* rare: laid out past the loop */
         gf<L> s;
         gf_add<L>(s, a, b, flipb, lr);
 #pragma unroll
