@@ -628,7 +628,7 @@ MP_HD void sqr_cols(const uint32_t* a, uint32_t* H, uint64_t& acc, uint32_t& pre
 }
 
 /* Square with the off-diagonal products formed once (n(n-1)/2
-   multiply-accumulates plus n squares); pays off from N >= 12 words. */
+   multiply-accumulates plus n squares). */
 template <int N, int K0>
 MP_HD void mant_sqr_sym(const uint32_t* a, uint32_t* H, uint32_t& S)
 {
@@ -727,9 +727,11 @@ constexpr int short_col()
 template <int N, bool SQR, int K0>
 MP_HD void mant_product(const uint32_t* a, const uint32_t* b, uint32_t* H, uint32_t& S)
 {
+    /* short squares by the doubling pass from 6 words (+4-10 % on 6- and
+       8-word frames; neutral at 4), exact ones by columns from 12 */
     if (SQR && N >= 12 && K0 == 0)
         mant_sqr_exact<N>(a, H, S);
-    else if (SQR && N >= 12)
+    else if (SQR && N >= 6 && K0 > 0)
         mant_sqr_sym<N, K0>(a, H, S);
     else
         mant_mul<N, K0>(a, SQR ? a : b, H, S);
