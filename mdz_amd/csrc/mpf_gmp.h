@@ -224,8 +224,54 @@ GF_HD void word_product(const limb (&U)[L], const limb (&V)[L], limb (&Z)[2 * L]
     }
 }
 
+/* words Q .. 2W-1 of the square of the W-word number UW (least significant
+   word first) into ZW: column Q of the off-diagonal sum O = sum over i < j of
+   UW[i] UW[j] 2^(32(i+j)) is product-scanned in multiply-accumulate groups;
+   once its word o is final, word Q of the square, 2 O + the diagonal
+   squares, follows from o, the previous word of O, one word of a diagonal
+   square and the carry c of that pass (mp_rn.h sqr_cols does the same for
+   the top words of a rounded square) */
+template <int W, int Q>
+GF_HD void word_sqr_cols(const uint32_t* UW, uint32_t* ZW, uint64_t& acc, uint32_t& prev,
+                         uint32_t& c, uint32_t& dhi)
+{
+    if constexpr (Q < 2 * W) {
+        uint32_t o;
+        if constexpr (Q <= 2 * W - 3) {
+            constexpr int i0 = Q < W ? 0 : Q - W + 1;
+            constexpr int i1 = (Q + 1) / 2 - 1;     /* largest i with i < Q - i */
+            constexpr int n = i1 >= i0 ? i1 - i0 + 1 : 0;
+            uint32_t c2 = 0;
+            if constexpr (n > 0) {
+                uint32_t ca[n], cb[n];
+#pragma unroll
+                for (int t = 0; t < n; ++t) {
+                    ca[t] = UW[i0 + t];
+                    cb[t] = UW[Q - i0 - t];
+                }
+                mdzg::mac_col<n, true>(ca, cb, acc, c2);
+            }
+            o = (uint32_t)acc;
+            acc = (acc >> 32) | ((uint64_t)c2 << 32);
+        } else {
+            o = (Q == 2 * W - 2) ? (uint32_t)acc : (uint32_t)(acc >> 32);
+        }
+        uint32_t d;
+        if constexpr ((Q & 1) == 0) {
+            const uint64_t sq = (uint64_t)UW[Q >> 1] * UW[Q >> 1];
+            d = (uint32_t)sq;
+            dhi = (uint32_t)(sq >> 32);
+        } else {
+            d = dhi;
+        }
+        ZW[Q] = mdzg::add_cc(mdzg::funnel_r(o, prev, 31u), d, c, c);
+        prev = o;
+        word_sqr_cols<W, Q + 1>(UW, ZW, acc, prev, c, dhi);
+    }
+}
+
 /* Z = U * U as word_product, with each off-diagonal word product formed
-   once and the column's off-diagonal sum doubled (pays from 12 words up) */
+   once (from 12 words up) */
 template <int L, int K>
 GF_HD void word_square(const limb (&U)[L], limb (&Z)[2 * L])
 {
@@ -238,41 +284,8 @@ GF_HD void word_square(const limb (&U)[L], limb (&Z)[2 * L])
     }
     uint32_t ZW[2 * W];
     uint64_t acc = 0;
-    uint32_t c2 = 0;
-#pragma unroll
-    for (int q = 0; q < 2 * W - 1; ++q) {
-        /* o = sum of UW[i] UW[q - i] over i < q - i (96 bits) */
-        uint64_t o = 0;
-        uint32_t o2 = 0;
-        bool first = true;
-#pragma unroll
-        for (int i = 0; i < W; ++i) {
-            const int j = q - i;
-            if (j > i && j < W) {
-                if (first)
-                    mdzg::mac96_first(UW[i], UW[j], o, o2);
-                else
-                    mdzg::mac96(UW[i], UW[j], o, o2);
-                first = false;
-            }
-        }
-        if (!first) {                  /* acc += 2 o */
-            const uint32_t d0 = (uint32_t)o << 1;
-            const uint32_t d1 = mdzg::funnel_r((uint32_t)(o >> 32), (uint32_t)o, 31u);
-            const uint32_t d2 = mdzg::funnel_r(o2, (uint32_t)(o >> 32), 31u);
-            uint32_t cc;
-            const uint32_t l0 = mdzg::add_cc((uint32_t)acc, d0, 0u, cc);
-            const uint32_t l1 = mdzg::add_cc((uint32_t)(acc >> 32), d1, cc, cc);
-            c2 = c2 + d2 + cc;
-            acc = ((uint64_t)l1 << 32) | l0;
-        }
-        if ((q & 1) == 0)
-            mdzg::mac96(UW[q >> 1], UW[q >> 1], acc, c2);
-        ZW[q] = (uint32_t)acc;
-        acc = (acc >> 32) | ((uint64_t)c2 << 32);
-        c2 = 0;
-    }
-    ZW[2 * W - 1] = (uint32_t)acc;
+    uint32_t prev = 0, c = 0, dhi = 0;
+    word_sqr_cols<W, 0>(UW, ZW, acc, prev, c, dhi);
 #pragma unroll
     for (int m = 0; m < 2 * L; ++m) {
         const int k = 2 * (2 * K - 1 - m);
