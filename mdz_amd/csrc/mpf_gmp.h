@@ -271,7 +271,7 @@ GF_HD void word_sqr_cols(const uint32_t* UW, uint32_t* ZW, uint64_t& acc, uint32
 }
 
 /* Z = U * U as word_product, with each off-diagonal word product formed
-   once (from 12 words up) */
+   once */
 template <int L, int K>
 GF_HD void word_square(const limb (&U)[L], limb (&Z)[2 * L])
 {
@@ -329,9 +329,14 @@ GF_HD void gf_mul(gf<L>& r, const gf<L>& u, const gf<L>& v, int lr)
        always suffices; the small buckets, which often hold fewer limbs than
        they store, take K = L - 2 when no lane of the wave needs more. */
     limb Z[2 * L];
-    if (SQR && 2 * (L - 1) >= 12)
+    /* squares by word_square from 6 words (4-limb bucket: +4 %, 6-limb
+       bucket: +7 % on the config 2 view against the plain product) */
+    const bool small = L <= 6 && L >= 3 && !gf_any((us > L - 2) | (vs > L - 2));
+    if (SQR && 2 * (L - 1) >= 6 && small)
+        word_square<L, (L >= 3 ? L - 2 : 1)>(U, Z);
+    else if (SQR && 2 * (L - 1) >= 6)
         word_square<L, L - 1>(U, Z);
-    else if (L <= 6 && L >= 3 && !gf_any((us > L - 2) | (vs > L - 2)))
+    else if (small)
         word_product<L, (L >= 3 ? L - 2 : 1)>(U, V, Z);
     else
         word_product<L, L - 1>(U, V, Z);
