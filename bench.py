@@ -433,7 +433,7 @@ def config4_mix(ctx, M, WL, np, n=48, fracs=(0.25, 0.5, 0.75), seed=0x5E4):
     ok = [x for x in sweep if x["p99_slowdown"] <= 2.0]
     best = max(ok, key=lambda x: x["gpix_per_s"]) if ok else None
     pr = out["per_request"]["saturation"]["gpix_per_s"]
-    out["north_star"] = dict(load=best["load"] if best else None,
+    out["batching_target"] = dict(load=best["load"] if best else None,
                              batched_gpix_per_s=best["gpix_per_s"] if best else None,
                              per_request_saturation_gpix_per_s=pr,
                              ratio=best["gpix_per_s"] / pr if best else None,
@@ -746,8 +746,8 @@ def main():
         log(f"rn1024: {res['rn1024']['kernel_gpix_per_s']:.3f} G pixel-iter/s (kernel)")
 
         # p50/p99 tile latency: continuous batching, 64x64 config 2 crops,
-        # against one launch per request (the north star's serving claim:
-        # >= 3x the per-request form's throughput at p99 <= 2x unloaded)
+        # against one launch per request (the serving target: >= 3x the
+        # per-request form's throughput at p99 <= 2x unloaded)
         tiles, _ = crop_tiles(WL, 300)
         unl = unloaded_latency(ctx, M, np, M.SERVE_BATCHED, tiles)
         sat = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, float("inf"))
@@ -773,7 +773,7 @@ def main():
                               per_request=dict(unloaded_p50_ms=unl_pr,
                                                saturation_tiles_per_s=sat_pr["tiles_per_s"],
                                                saturation_gpix_per_s=sat_pr["gpix_per_s"]),
-                              north_star=dict(
+                              batching_target=dict(
                                   load=best["load"] if best else None,
                                   p99_over_unloaded=best["p99_over_unloaded"] if best else None,
                                   batched_gpix_per_s=best["gpix_per_s"] if best else None,
@@ -782,7 +782,7 @@ def main():
                                   target="ratio >= 3 at p99 <= 2x unloaded latency"))
         log(f"latency: unloaded {unl:.2f} ms; half load p50 {half['p50_ms']:.2f} "
             f"p99 {half['p99_ms']:.2f} ms; per-request saturation {sat_pr['gpix_per_s']:.1f} G; "
-            f"north star ratio {res['latency']['north_star']['ratio']}")
+            f"batching target ratio {res['latency']['batching_target']['ratio']}")
 
         # config 3: 256 mixed tiles, persistent batch against one launch per tile
         c3 = config3_batched(ctx, M, WL, np)
