@@ -654,7 +654,7 @@ __device__ __forceinline__ bool rn_step(rn_state<N>& s, int fractal, int32_t wz,
     t.s = (fractal == F_BURNING_SHIP) ? 0u : t.s;            /* BS: |t1|         */
     twice<N>(t);                                             /* t1 *= 2          */
     add_rn<N>(s.wim, t, s.c.get_im(), 0u, k);                /* wim = t1 + c_im  */
-    add_rn<N, SUB_NONNEG>(t, s.wre2, s.wim2, 1u, k);         /* t1 = wre2 - wim2 */
+    add_rn<N, SUB_NONNEG, FULL>(t, s.wre2, s.wim2, 1u, k);   /* t1 = wre2 - wim2 */
     const bool ab = fractal == F_GENERALIZED_CELTIC || (fractal == F_VARIANT && (wz & 1));
     t.s = ab ? 0u : t.s;                                     /* Celtic: |t1|     */
     add_rn<N>(s.wre, t, s.c.get_re(), 0u, k);                /* wre = t1 + c_re  */

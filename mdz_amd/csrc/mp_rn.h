@@ -799,7 +799,7 @@ MP_HD void mul_rn(mp<N>& r, const mp<N>& a, const mp<N>& b, int kclr)
 /* ------------------------------------------------------------------ */
 enum { ADD_ANY = 0, SUB_NONNEG = 1 };
 
-template <int N, int KIND = ADD_ANY>
+template <int N, int KIND = ADD_ANY, bool WORD_ALIGNED = false>
 MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb, int kclr)
 {
     constexpr bool SUBNN = KIND == SUB_NONNEG;
@@ -830,6 +830,25 @@ MP_HD void add_rn(mp<N>& r, const mp<N>& a_in, const mp<N>& b_in, uint32_t flipb
     /* align B right by d bits into N+1 words X (X[0] = guard word) + sticky */
     uint32_t d = (uint32_t)(e - eb);
     if (d > 32u * (N + 1)) d = 32u * (N + 1);
+    /* |B| < 2^(e-d) is below a quarter of A's last place when d >= p + 2
+       (p <= 32N): A - B lies within that quarter of A, nearer A than any
+       other p-bit value (below a power of two the spacing halves, still more
+       than the quarter), so RN_p(A - B) = A.  Near the real axis wim2 is
+       that small against wre2 in every iteration, and the squares'
+       difference skips its alignment, carry chain and rounding.  Only the
+       squares' difference, only from 6 words (the vote in every addition of
+       the 4-word loop cost config 2 4 %) and only in the word-aligned
+       kernels (WORD_ALIGNED: p = 32N), whose register budget it leaves as
+       it was; the ragged 8- and 16-word kernels spilled more with it. */
+    if constexpr (SUBNN && WORD_ALIGNED && N >= 6) {
+        if (wave_all(d >= 32u * N + 2u)) {
+#pragma unroll
+            for (int j = 0; j < N; ++j) r.m[j] = A[j];
+            r.e = e;
+            r.s = sa;
+            return;
+        }
+    }
     const uint32_t ws = d >> 5, bs = d & 31u;
     uint32_t X[N + 1];
     X[0] = funnel_r(B[0], 0u, bs);

@@ -44,6 +44,8 @@ __global__ __launch_bounds__(256) void k_check(int op, int kclr, int n, const ui
         mul_rn<N, false>(z, x, y, kclr);
     else if (op == 3)
         mul_rn<N, true>(z, x, x, kclr);
+    else if (kclr == 0)   /* as the word-aligned kernels call it */
+        add_rn<N, SUB_NONNEG, true>(z, x, y, 1u, kclr);
     else
         add_rn<N, SUB_NONNEG>(z, x, y, 1u, kclr);
 #pragma unroll
