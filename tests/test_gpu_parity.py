@@ -279,6 +279,42 @@ def test_batch_modes_agree_with_oracle(ctx, oracle):
         assert _eq(x, y) and _eq(x, r)
 
 
+def test_lazy_squares_with_refills(ctx, oracle):
+    """The 1024-bit word-aligned loop keeps only an exponent bound for wim^2
+    while it lies 32N+2 bits below wre^2 on every lane of a wave.  Tiles A
+    are 64 one-row tiles k e-200 above the real axis (k = 1..64), from
+    c = 0.2501 to 0.2597: every lane is in that state until its pixel
+    escapes, after 29 to 311 iterations, and neighbours in a wave escape at
+    different iterations.  With one block per CU (65,536 lanes, the A tiles'
+    pixel count) and a refill every iteration, lanes that finish take pixels
+    of tile B, off the axis, into waves whose other lanes still hold bounds,
+    which then square those lanes exactly.  Every tile equals the oracle, in
+    the persistent batch and one launch per tile."""
+    specs, refs = [], []
+    for k in range(64):
+        rect = ("0.2501", "0.2597", "%de-200" % (k + 1), "0.0096")
+        specs.append((M.make_view(rect, precision=1024, depth=2000, width=1024, height=1),
+                      (0, 1), (0, 1024)))
+        refs.append(oracle.render(rect, precision=1024, depth=2000, width=1024, height=1,
+                                  threads=16))
+    rect_b = oracle.rect_from_center("0.2801", "0.0102", "1e-3", 1024, 64, 64)
+    specs.append((M.make_view(rect_b, precision=1024, depth=600, width=64, height=64),
+                  (0, 64), (0, 64)))
+    refs.append(oracle.render(rect_b, precision=1024, depth=600, width=64, height=64,
+                              threads=16))
+    assert refs[0].min() > 0 and len(np.unique(refs[0])) > 100   # every A pixel escapes
+    kid = M.kernel_id(M.PATH_MPFR, 1024)
+    keep = ctx.tune_get(kid)[:2]
+    ctx.tune_set(kid, blocks_per_cu=1, chunk=1)
+    try:
+        a, _ = ctx.render_batch(specs, M.BATCH_PERSISTENT)
+    finally:
+        ctx.tune_set(kid, blocks_per_cu=keep[0], chunk=keep[1])
+    b, _ = ctx.render_batch(specs, M.BATCH_PER_TILE)
+    for x, y, r in zip(a, b, refs):
+        assert _eq(x, r) and _eq(y, r)
+
+
 @pytest.mark.parametrize("chunk", [1, 3, 4, 5, 256])
 def test_fp64_runs_mixed_depths(ctx, oracle, chunk):
     """FP64 persistent runs: one launch whose waves hold lanes of small tiles
