@@ -940,6 +940,7 @@ __global__ __launch_bounds__(256, (rn_min_waves_slice<N, FULL>())) void k_rn_sli
                     load_mp<N>(s.wim, st + MDZG_VS(N));
                     load_mp<N>(s.wre2, st + 2 * MDZG_VS(N));
                     load_mp<N>(s.wim2, st + 3 * MDZG_VS(N));
+                    s.lazy = false;
                     mp<N> cr, ci;
                     if (tp->family == 1) {
                         load_mp<N>(cr, tp->jc);
@@ -965,12 +966,17 @@ __global__ __launch_bounds__(256, (rn_min_waves_slice<N, FULL>())) void k_rn_sli
             ++wz;
             --left;
             --lb;
-            const bool esc = rn_step<N, FULL>(s, fractal, wz, kclr);
+            const bool esc = rn_step<N, FULL, true>(s, fractal, wz, kclr);
             if (esc || wz >= depth) {
                 put_count(tp->out + local, esc ? (uint32_t)wz : 0u);
                 atomicSub(tp->remaining, 1u);
                 active = false;
             } else if (left == 0 || lb == 0) {
+                /* a lazy wim2 is only a bound: the saved state holds the
+                   exact square (wim is the value it squares) */
+                if (s.lazy)
+                    mul_rn<N, true>(s.wim2, s.wim, s.wim, FULL ? 0 : kclr);
+                s.lazy = false;
                 uint32_t* st = tp->state + (size_t)local * 4 * MDZG_VS(N);
                 store_mp<N>(st, s.wre);
                 store_mp<N>(st + MDZG_VS(N), s.wim);

@@ -221,3 +221,30 @@ def test_server_staging_failure_fails_only_that_request(ctx, monkeypatch):
             continue
         assert infos[i]["status"] == 2, i
         assert _eq(outs[i], ref), i
+
+
+def test_server_lazy_squares_across_slices(ctx, oracle):
+    """1024-bit rows just above the real axis, where the word-aligned loop
+    keeps only an exponent bound for wim^2, served in slices of 7 iterations
+    next to an off-axis request of the same class: a slice that stops on a
+    lane with a bound saves the exact square, and resumed lanes mix with
+    lanes holding bounds.  Counts equal the oracle."""
+    reqs, refs = [], []
+    for k in range(8):
+        rect = ("0.2501", "0.2597", "%de-200" % (k + 1), "0.0096")
+        reqs.append((M.make_view(rect, precision=1024, depth=2000, width=1024, height=1),
+                     (0, 1), (0, 1024)))
+        refs.append(oracle.render(rect, precision=1024, depth=2000, width=1024, height=1,
+                                  threads=16))
+    rect_b = oracle.rect_from_center("0.2801", "0.0102", "1e-3", 1024, 64, 64)
+    reqs.append((M.make_view(rect_b, precision=1024, depth=600, width=64, height=64),
+                 (0, 16), (0, 64)))
+    refs.append(oracle.render(rect_b, precision=1024, depth=600, width=64, height=64,
+                              lines=(0, 16, 1), threads=16))
+    with M.Server(ctx, slice_iters=7) as srv:
+        ids = [srv.submit(v, l, c) for v, l, c in reqs]
+        outs = [srv.wait(i) for i in ids]
+        assert srv.stats()["slices"] > 20
+    for (out, info), ref in zip(outs, refs):
+        assert info["status"] == 2
+        assert _eq(out, ref)
