@@ -315,6 +315,39 @@ def test_lazy_squares_with_refills(ctx, oracle):
         assert _eq(x, r) and _eq(y, r)
 
 
+@pytest.mark.parametrize("prec", [384, 512, 768])
+def test_lazy_squares_other_word_counts(ctx, oracle, prec):
+    """The exponent-bound squares at 12, 16 and 24 words: 64 one-row tiles
+    k e-200 above the real axis (wim^2 some 1300 bits below wre^2, beyond
+    32N + 2 at every word count here) and an off-axis tile, in one
+    persistent launch with one block per CU and a refill every iteration, so
+    lanes of the off-axis tile join waves whose other lanes hold bounds.
+    Every tile equals the oracle, in both batch forms."""
+    specs, refs = [], []
+    for k in range(64):
+        rect = ("0.2501", "0.2597", "%de-200" % (k + 1), "0.0096")
+        specs.append((M.make_view(rect, precision=prec, depth=2000, width=1024, height=1),
+                      (0, 1), (0, 1024)))
+        refs.append(oracle.render(rect, precision=prec, depth=2000, width=1024, height=1,
+                                  threads=16))
+    rect_b = oracle.rect_from_center("0.2801", "0.0102", "1e-3", prec, 64, 64)
+    specs.append((M.make_view(rect_b, precision=prec, depth=600, width=64, height=64),
+                  (0, 64), (0, 64)))
+    refs.append(oracle.render(rect_b, precision=prec, depth=600, width=64, height=64,
+                              threads=16))
+    assert refs[0].min() > 0 and len(np.unique(refs[0])) > 100
+    kid = M.kernel_id(M.PATH_MPFR, prec)
+    keep = ctx.tune_get(kid)[:2]
+    ctx.tune_set(kid, blocks_per_cu=1, chunk=1)
+    try:
+        a, _ = ctx.render_batch(specs, M.BATCH_PERSISTENT)
+    finally:
+        ctx.tune_set(kid, blocks_per_cu=keep[0], chunk=keep[1])
+    b, _ = ctx.render_batch(specs, M.BATCH_PER_TILE)
+    for x, y, r in zip(a, b, refs):
+        assert _eq(x, r) and _eq(y, r)
+
+
 @pytest.mark.parametrize("chunk", [1, 3, 4, 5, 256])
 def test_fp64_runs_mixed_depths(ctx, oracle, chunk):
     """FP64 persistent runs: one launch whose waves hold lanes of small tiles
