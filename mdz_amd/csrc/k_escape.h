@@ -666,11 +666,14 @@ __device__ __forceinline__ void rn_init(rn_state<N>& s, const mdzg_dtile& t, int
    and e(wre^2) >= 2 e(wre) - 1.  Near the real axis it took 1024-bit frames
    from 12.4 to 15.6 G pixel-iter/s; off the axis its votes and state cost
    2-3 % from 12 words and 3-9 % at 6 and 8, which keep the plain loop. */
+template <int N, bool FULL>
+__device__ constexpr bool rn_lazy() { return FULL && N >= 12; }
+
 template <int N, bool FULL, bool LAZY = false>
 __device__ __forceinline__ bool rn_step(rn_state<N>& s, int fractal, int32_t wz, int kclr)
 {
     const int k = FULL ? 0 : kclr;
-    constexpr bool LZ = LAZY && FULL && N >= 12;
+    constexpr bool LZ = LAZY && rn_lazy<N, FULL>();
     constexpr int32_t GAP = 32 * N + 2;
     if constexpr (LZ) {
         if (__builtin_expect(wave_any(s.lazy) &&
@@ -974,9 +977,11 @@ __global__ __launch_bounds__(256, (rn_min_waves_slice<N, FULL>())) void k_rn_sli
             } else if (left == 0 || lb == 0) {
                 /* a lazy wim2 is only a bound: the saved state holds the
                    exact square (wim is the value it squares) */
-                if (s.lazy)
-                    mul_rn<N, true>(s.wim2, s.wim, s.wim, FULL ? 0 : kclr);
-                s.lazy = false;
+                if constexpr (rn_lazy<N, FULL>()) {
+                    if (s.lazy)
+                        mul_rn<N, true>(s.wim2, s.wim, s.wim, 0);
+                    s.lazy = false;
+                }
                 uint32_t* st = tp->state + (size_t)local * 4 * MDZG_VS(N);
                 store_mp<N>(st, s.wre);
                 store_mp<N>(st + MDZG_VS(N), s.wim);
