@@ -336,6 +336,17 @@ def _brief(r):
     return {k: v for k, v in r.items() if k != "latencies_ms"}
 
 
+def saturation_run(ctx, M, np, mode, tiles):
+    """Saturation throughput at steady state: one untimed run of the whole
+    request set, then the timed one.  A saturation run that follows a mostly
+    idle phase (the unloaded latencies, one request at a time) or a small
+    warm-up ran 10-15 % slower on 300 config 2 crops than one that follows
+    another busy run (86-89 against 95-99 G pixel-iter/s, graphs on or
+    off)."""
+    serve_run(ctx, M, np, mode, tiles, float("inf"))
+    return serve_run(ctx, M, np, mode, tiles, float("inf"))
+
+
 def unloaded_latency(ctx, M, np, mode, tiles, reps=15):
     srv = M.Server(ctx, mode=mode)
     lat = []
@@ -354,7 +365,7 @@ def config4_serving(ctx, M, WL, np, fracs=(0.25, 0.5, 0.75, 0.9), n=300):
     serve_run(ctx, M, np, M.SERVE_BATCHED, tiles[:60], float("inf"))     # warm-up
     for name, mode in (("batched", M.SERVE_BATCHED), ("per_request", M.SERVE_PER_REQUEST)):
         unl = unloaded_latency(ctx, M, np, mode, tiles)
-        sat = serve_run(ctx, M, np, mode, tiles, float("inf"))
+        sat = saturation_run(ctx, M, np, mode, tiles)
         log(f"serve {name}: unloaded {unl:.2f} ms, saturation {sat['tiles_per_s']:.0f} tiles/s "
             f"{sat['gpix_per_s']:.1f} G pixel-iter/s")
         sweep = []
@@ -376,7 +387,7 @@ def config4_serving(ctx, M, WL, np, fracs=(0.25, 0.5, 0.75, 0.9), n=300):
     # the batched form with each slice's launches made one by one (no graphs)
     ctx.option_set(M.OPT_GRAPHS, 0)
     unl0 = unloaded_latency(ctx, M, np, M.SERVE_BATCHED, tiles)
-    sat0 = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, float("inf"))
+    sat0 = saturation_run(ctx, M, np, M.SERVE_BATCHED, tiles)
     half0 = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, 0.5 * sat0["tiles_per_s"])
     ctx.option_set(M.OPT_GRAPHS, 1)
     out["batched_without_graphs"] = dict(unloaded_ms=unl0, saturation=_brief(sat0),
@@ -756,9 +767,9 @@ def main():
         # per-request form's throughput at p99 <= 2x unloaded)
         tiles, _ = crop_tiles(WL, 300)
         unl = unloaded_latency(ctx, M, np, M.SERVE_BATCHED, tiles)
-        sat = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, float("inf"))
+        sat = saturation_run(ctx, M, np, M.SERVE_BATCHED, tiles)
         unl_pr = unloaded_latency(ctx, M, np, M.SERVE_PER_REQUEST, tiles)
-        sat_pr = serve_run(ctx, M, np, M.SERVE_PER_REQUEST, tiles, float("inf"))
+        sat_pr = saturation_run(ctx, M, np, M.SERVE_PER_REQUEST, tiles)
         loads = []
         for f in (0.5, 0.6, 0.7, 0.8, 0.9):
             r = serve_run(ctx, M, np, M.SERVE_BATCHED, tiles, f * sat["tiles_per_s"])
